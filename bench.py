@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark: SCFT iterations/s of the PmmmDCT path at 128^3, Ns = 200.
+
+One "step" is one full SCFT iteration (crysSCFT_sample.cpp:476-508): 400
+contour steps of the two propagator chains, the densities, Q, the free
+energy and the mixing update, all on the GPU with the fields resident in HBM.
+Workload: f_A = 0.30, chiN = 25, cubic L = 4.2 Rg, 128^3 logical grid
+(64^3 reduced), ds = 1/200 (NsA = 60, NsB = 140), BCC-seeded Pmmm fields,
+simple mixing 0.1.
+
+Prints ONE JSON line.  The roofline block is for the contour step (the three
+axis passes of both chains), timed with HIP events on the solver's stream;
+the CPU baseline is the oracle (a C restatement of the reference) timed on
+this host on one SCFT iteration of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_LOG = 128
+FA, XN, L, DS, ACC = 0.30, 25.0, 4.2, 1.0 / 200, 0.1
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_iteration(n, b, NsA, NsB):
+    """Algorithmic HBM bytes of one SCFT iteration: 2 Ns' contour steps of
+    3 n b each, the phi pass over both histories, 22 n doubles of field work."""
+    Ns = NsA + NsB
+    return 2 * Ns * 3 * n * b + 2 * (Ns + 2) * n * b + 22 * n * 8
+
+
+def run_gpu(precision, steps, warmup, wA, wB):
+    import crysscft_sample_amd as C
+
+    solver = C.SolverPmmmDCT((N_LOG,) * 3, precision=precision)
+    run = C.SCFT(solver, FA, DS, XN, L, ACC)
+    run.set_fields(wA, wB)
+    if warmup:
+        run.run(warmup, trace=True)
+    C.lib().crys_device_synchronize()
+    t0 = time.perf_counter()
+    tr = run.run(steps, trace=True)
+    C.lib().crys_device_synchronize()
+    dt = time.perf_counter() - t0
+    ms_step, ms_pass = run.time_step(reps=200)
+    out = dict(iter_s=steps / dt, ms_per_iter=dt / steps * 1e3, trace=tr, NsA=run.NsA,
+               NsB=run.NsB, ms_per_contour_step=ms_step, ms_per_pass=ms_pass)
+    run.close()
+    solver.close()
+    return out
+
+
+def cpu_baseline(wA, wB, threads):
+    """The oracle on one SCFT iteration of the same workload (bounded sample)."""
+    from oracle import oracle as O
+
+    O.set_num_threads(threads)
+    s = O.Solver("PmmmDCT", (N_LOG,) * 3)
+    t0 = time.perf_counter()
+    tr, *_ = s.scft(FA, DS, XN, L, ACC, 1, wA, wB)
+    dt = time.perf_counter() - t0
+    return 1.0 / dt, tr
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-propagator leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
+              "(the path shards by independent SCFT instances, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from crysscft_sample_amd import fields as F
+
+    wfA, wfB = F.bcc_fields((N_LOG,) * 3, L, FA, XN)
+    wA, wB = F.extract(wfA, "PmmmDCT"), F.extract(wfB, "PmmmDCT")
+    n = wA.size
+
+    print(f"bench: FP64 PmmmDCT 128^3 Ns=200, {args.warmup} warmup + {args.steps} timed",
+          file=sys.stderr, flush=True)
+    f64 = run_gpu("fp64", args.steps, args.warmup, wA, wB)
+    mixed = None
+    if not args.no_mixed:
+        print("bench: FP32 propagators + FP64 reductions", file=sys.stderr, flush=True)
+        mixed = run_gpu("fp32", args.steps, args.warmup, wA, wB)
+
+    NsA, NsB = f64["NsA"], f64["NsB"]
+    b_iter64 = bytes_per_iteration(n, 8, NsA, NsB)
+    # contour step of both chains: read q_s, read E, write q_s+1 per chain
+    step_bytes = 2 * 3 * n * 8
+    achieved = step_bytes / (f64["ms_per_contour_step"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "axis_pass_kernel<double> z+y+x (one contour step, both chains)",
+                "ms_per_launch": round(f64["ms_per_pass"], 6),
+                "bytes_per_launch": step_bytes / 3}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        print(f"bench: CPU baseline (oracle, {threads} threads, 1 iteration)", file=sys.stderr,
+              flush=True)
+        v, tr_cpu = cpu_baseline(wA, wB, threads)
+        dF = abs(f64["trace"][0, 0] / tr_cpu[0, 0] - 1)
+        cpu = {"value": round(v, 4), "unit": "iter/s", "cores": threads, "kind": "port",
+               "sample": "1 SCFT iteration (400 PmmmDCT diffusions + driver) of the same "
+                         "128^3 Ns=200 workload, own radix DCT, OpenMP over lines",
+               "iter0_rel_dF_vs_gpu": dF}
+
+    line = {
+        "metric": "SCFT iter/s at 128^3 Ns=200 PmmmDCT: mixed vs FP64 GPU; HBM GB/s",
+        "value": round(f64["iter_s"], 3),
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f64["ms_per_iter"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (BCC-seeded Pmmm fields, splitmix64 seed 20241203)",
+        "config": {"workload": "AB diblock SCFT, PmmmDCT, 128^3 logical / 64^3 reduced, "
+                               "fA=0.30 chiN=25 L=4.2 ds=1/200 (NsA=60, NsB=140), mixing 0.1",
+                   "solver": "PmmmDCT", "grid": [N_LOG] * 3, "NsA": NsA, "NsB": NsB},
+        "hbm_gbs": round(b_iter64 * f64["iter_s"] / 1e9, 2),
+        "bytes_per_iter": b_iter64,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if mixed is not None:
+        b_iter32 = bytes_per_iteration(n, 4, NsA, NsB) + 0  # FP64 field work unchanged
+        dF = float(np.abs(mixed["trace"][:, 0] / f64["trace"][:, 0] - 1).max())
+        line["mixed"] = {"iter_s": round(mixed["iter_s"], 3),
+                         "speedup_vs_fp64": round(mixed["iter_s"] / f64["iter_s"], 3),
+                         "hbm_gbs": round(b_iter32 * mixed["iter_s"] / 1e9, 2),
+                         "max_rel_dF_vs_fp64_gpu": dF,
+                         "ms_per_contour_step": round(mixed["ms_per_contour_step"], 6)}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

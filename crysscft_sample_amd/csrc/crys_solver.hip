@@ -1,0 +1,774 @@
+// libcrys_solver: the C ABI of include/crys_solver.h on gfx950.
+//
+// Host side of the drop-in for CrysPSSolverBase (solver_base.h:8-51) and its
+// three solvers, plus the device-resident SCFT iteration of
+// diblock_scft_simple_iterate (crysSCFT_sample.cpp:426-531).
+#include "crys_solver.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "axis_pass.hpp"
+#include "field_ops.hpp"
+#include "operators.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(e_ == hipErrorOutOfMemory ? CRYS_ERR_OUT_OF_MEMORY : CRYS_ERR_HIP,      \
+                  "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define CHK(expr)                 \
+  do {                            \
+    int r_ = (expr);              \
+    if (r_ != CRYS_OK) return r_; \
+  } while (0)
+
+constexpr int kTraceCap = 1024;
+
+unsigned grid_for(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  return (unsigned)(b ? b : 1);
+}
+
+}  // namespace
+
+// One cached operator set: the three per-axis matrices for one ds value.
+struct OpSet {
+  double ds;
+  void* dmat[3];  // device, element type of the solver's precision
+};
+
+struct crys_solver {
+  int kind = 0, precision = 0, flags = 0;
+  size_t nlog[3] = {0, 0, 0}, nphy[3] = {0, 0, 0};
+  size_t n_log = 0, n_phy = 0;
+  double cell[6] = {4, 4, 4, 0, 0, 0};  // solver_base.h:43
+  double ds_current = 0.0;              // solver_base.h:46
+  double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  hipStream_t stream = nullptr;
+  double* iomat = nullptr;  // device, n_phy doubles
+  void* work = nullptr;     // device, n_phy elements of the precision
+  void* scratch = nullptr;  // device, n_phy elements of the precision
+  std::vector<OpSet> ops;
+  int cur = -1;
+};
+
+struct crys_scft {
+  crys_solver* sv = nullptr;
+  double fA = 0, fB = 0, ds = 0, xN = 0, acc = 0, L[3] = {0, 0, 0};
+  size_t NsA = 0, NsB = 0, Ns = 0, n = 0;
+  double *wA = nullptr, *wB = nullptr, *ksi = nullptr, *phiA = nullptr, *phiB = nullptr;
+  void *EA = nullptr, *EB = nullptr, *qf = nullptr, *qb = nullptr, *scr[2] = {nullptr, nullptr};
+  double *qpart = nullptr, *epart = nullptr, *scal = nullptr, *trace = nullptr;
+  int* counter = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* graph_mats[3] = {nullptr, nullptr, nullptr};
+  bool fields_set = false;
+};
+
+namespace {
+
+size_t elem_size(int precision) { return precision == CRYS_FP32 ? sizeof(float) : sizeof(double); }
+
+void free_ops(crys_solver* h) {
+  for (auto& o : h->ops)
+    for (int a = 0; a < 3; ++a)
+      if (o.dmat[a]) (void)hipFree(o.dmat[a]);
+  h->ops.clear();
+  h->cur = -1;
+}
+
+int build_ops(crys_solver* h, double ds, OpSet* out) {
+  out->ds = ds;
+  for (int a = 0; a < 3; ++a) out->dmat[a] = nullptr;
+  for (int a = 0; a < 3; ++a) {
+    std::vector<double> At;
+    const int M = (int)h->nphy[a];
+    if (h->kind == CRYS_PMMM_DCT) At = crys::pmmm_axis_operator(M, h->cell[a], ds);
+    else if (h->kind == CRYS_P1) At = crys::p1_axis_operator((int)h->nlog[a], h->cell[a], ds);
+    else At = crys::m3_axis_operator((int)h->nlog[a], h->cell[a], ds);
+    const size_t cnt = (size_t)M * M;
+    HIPCHK(hipMalloc(&out->dmat[a], cnt * elem_size(h->precision)));
+    if (h->precision == CRYS_FP32) {
+      std::vector<float> f(At.begin(), At.end());
+      HIPCHK(hipMemcpy(out->dmat[a], f.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(hipMemcpy(out->dmat[a], At.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  return CRYS_OK;
+}
+
+// One contour step E D E (or a bare D with E == nullptr) for 1 or 2 chains:
+// z pass in -> scr (pre E), y pass in place on scr, x pass scr -> out (post E).
+template <typename T>
+int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const in[2],
+                T* const out[2], T* const scr[2], const T* const E[2], hipStream_t s) {
+  crys::PassParams<T> p{};
+  p.X = (int)h->nphy[0];
+  p.Y = (int)h->nphy[1];
+  p.Z = (int)h->nphy[2];
+  for (int c = 0; c < nch; ++c) p.c[c] = {in[c], scr[c], E[c], nullptr};
+  p.mat = (const T*)op.dmat[2];
+  HIPCHK(crys::launch_axis_pass<T>(2, p, nch, s));
+  for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], scr[c], nullptr, nullptr};
+  p.mat = (const T*)op.dmat[1];
+  HIPCHK(crys::launch_axis_pass<T>(1, p, nch, s));
+  for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], out[c], nullptr, E[c]};
+  p.mat = (const T*)op.dmat[0];
+  HIPCHK(crys::launch_axis_pass<T>(0, p, nch, s));
+  return CRYS_OK;
+}
+
+int check_handle(const crys_solver* h) {
+  if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null solver handle");
+  return CRYS_OK;
+}
+
+int diffusion_impl(crys_solver* h) {
+  if (h->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "diffusion() before set_contour_step()");
+  const OpSet& op = h->ops[h->cur];
+  const size_t n = h->n_phy;
+  if (h->precision == CRYS_FP64) {
+    double* io = h->iomat;
+    const double* in[2] = {io, nullptr};
+    double* out[2] = {io, nullptr};
+    double* scr[2] = {io, nullptr};
+    const double* E[2] = {nullptr, nullptr};
+    return launch_step<double>(h, op, 1, in, out, scr, E, h->stream);
+  }
+  float* w = (float*)h->work;
+  hipLaunchKernelGGL((crys::convert_kernel<float, double>), dim3(grid_for(n)), dim3(256), 0,
+                     h->stream, w, (const double*)h->iomat, n);
+  HIPCHK(hipGetLastError());
+  const float* in[2] = {w, nullptr};
+  float* out[2] = {w, nullptr};
+  float* scr[2] = {w, nullptr};
+  const float* E[2] = {nullptr, nullptr};
+  CHK(launch_step<float>(h, op, 1, in, out, scr, E, h->stream));
+  hipLaunchKernelGGL((crys::convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0,
+                     h->stream, h->iomat, (const float*)w, n);
+  HIPCHK(hipGetLastError());
+  return CRYS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* crys_last_error(void) { return g_err.c_str(); }
+
+const char* crys_version(void) { return "crysscft_sample_amd 0.1 (gfx950)"; }
+
+int crys_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int crys_device_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return CRYS_OK;
+}
+
+int crys_solver_create(int kind, const size_t grid_log[3], const double translational_part[9],
+                       int precision, int flags, crys_solver_t** out) {
+  if (!out || !grid_log) return fail(CRYS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (kind != CRYS_P1 && kind != CRYS_3M && kind != CRYS_PMMM_DCT)
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "unknown solver kind %d", kind);
+  if (precision != CRYS_FP64 && precision != CRYS_FP32)
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
+  const size_t Nx = grid_log[0], Ny = grid_log[1], Nz = grid_log[2];
+  if (kind == CRYS_P1) {
+    // solver_p1.h:101-108
+    if (Nx == 0 || Ny == 0 || Nz == 0)
+      return fail(CRYS_ERR_BAD_GRID, "ERROR: Bad Grid Size For Group Triclinic P1");
+    if (Nz % 8 != 0)
+      return fail(CRYS_ERR_BAD_GRID,
+                  "ERROR: Bad Grid Size For Group Triclinic P1, alignment cannot be guaranteed");
+  } else {
+    // solver_3m.h:81-92, solver_Pmmm.h:71-82
+    if (Nx % 2 || Ny % 2 || Nz % 2)
+      return fail(CRYS_ERR_BAD_GRID, "ERROR: Grid Size Must Be Even For %s",
+                  kind == CRYS_3M ? "Generalized 3m symmetry" : "Group Pmmm");
+    if (Nx == 0 || Ny == 0 || Nz == 0)
+      return fail(CRYS_ERR_BAD_GRID, "ERROR: Bad Grid Size For %s",
+                  kind == CRYS_3M ? "Generalized 3m symmetry" : "Group Pmmm");
+    if ((Nz / 2) % 8 != 0)
+      return fail(CRYS_ERR_BAD_GRID, "ERROR: Bad Grid Size For %s, Alignment Cannot Be Guaranteed",
+                  kind == CRYS_3M ? "Generalized 3m symmetry" : "Group Pmmm");
+  }
+  crys_solver* h = new (std::nothrow) crys_solver();
+  if (!h) return fail(CRYS_ERR_OUT_OF_MEMORY, "host allocation failed");
+  h->kind = kind;
+  h->precision = precision;
+  h->flags = flags;
+  for (int a = 0; a < 3; ++a) {
+    h->nlog[a] = grid_log[a];
+    h->nphy[a] = kind == CRYS_P1 ? grid_log[a] : grid_log[a] / 2;
+  }
+  if (kind == CRYS_3M && translational_part)
+    for (int i = 0; i < 9; ++i) h->g[i] = translational_part[i];
+  h->n_log = Nx * Ny * Nz;
+  h->n_phy = h->nphy[0] * h->nphy[1] * h->nphy[2];
+  for (int a = 0; a < 3; ++a)
+    if (crys::pass_class((int)h->nphy[a]) == 0) {
+      delete h;
+      return fail(CRYS_ERR_UNSUPPORTED, "physical grid length %zu exceeds 128 on axis %d",
+                  grid_log[a], a);
+    }
+  if (kind == CRYS_3M)
+    for (int i = 0; i < 9; ++i)
+      if (h->g[i] != 0.0) {
+        delete h;
+        return fail(CRYS_ERR_UNSUPPORTED,
+                    "Solver3m with non-zero translational parts is not implemented on the GPU");
+      }
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&h->iomat, h->n_phy * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&h->work, h->n_phy * elem_size(precision));
+  if (e == hipSuccess) e = hipMalloc(&h->scratch, h->n_phy * elem_size(precision));
+  if (e == hipSuccess) e = hipMemset(h->iomat, 0, h->n_phy * sizeof(double));
+  if (e != hipSuccess) {
+    crys_solver_destroy(h);
+    return fail(e == hipErrorOutOfMemory ? CRYS_ERR_OUT_OF_MEMORY : CRYS_ERR_HIP,
+                "solver allocation failed: %s", hipGetErrorString(e));
+  }
+  *out = h;
+  return CRYS_OK;
+}
+
+int crys_solver_destroy(crys_solver_t* h) {
+  if (!h) return CRYS_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_ops(h);
+  if (h->iomat) (void)hipFree(h->iomat);
+  if (h->work) (void)hipFree(h->work);
+  if (h->scratch) (void)hipFree(h->scratch);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return CRYS_OK;
+}
+
+int crys_set_cell_para(crys_solver_t* h, const double c[6]) {
+  CHK(check_handle(h));
+  if (!c) return fail(CRYS_ERR_INVALID_ARGUMENT, "null cell_para");
+  if (h->kind != CRYS_P1) {
+    // solver_3m.h:117, solver_Pmmm.h:100
+    if (c[0] == h->cell[0] && c[1] == h->cell[1] && c[2] == h->cell[2] && c[3] == 0 && c[4] == 0 &&
+        c[5] == 0)
+      return CRYS_OK;
+  }
+  if (!(c[0] > 0 && c[1] > 0 && c[2] > 0))
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "cell lengths must be positive");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->kind == CRYS_PMMM_DCT && !(h->flags & CRYS_FLAG_PMMM_ORTHORHOMBIC)) {
+    const double nc[6] = {c[0], c[0], c[0], 0, 0, 0};  // solver_Pmmm.h:101
+    memcpy(h->cell, nc, sizeof(nc));
+  } else {
+    const double nc[6] = {c[0], c[1], c[2], 0, 0, 0};
+    memcpy(h->cell, nc, sizeof(nc));
+  }
+  free_ops(h);
+  h->ds_current = 0.0;
+  return CRYS_OK;
+}
+
+int crys_set_contour_step(crys_solver_t* h, double ds) {
+  CHK(check_handle(h));
+  if (!(ds > 0) || !std::isfinite(ds)) return fail(CRYS_ERR_INVALID_ARGUMENT, "ds must be > 0");
+  if (!h->ops.empty() && ds == h->ds_current && h->cur >= 0) return CRYS_OK;
+  h->ds_current = ds;
+  for (size_t i = 0; i < h->ops.size(); ++i)
+    if (h->ops[i].ds == ds) {
+      h->cur = (int)i;
+      return CRYS_OK;
+    }
+  OpSet o;
+  int r = build_ops(h, ds, &o);
+  if (r != CRYS_OK) {
+    for (int a = 0; a < 3; ++a)
+      if (o.dmat[a]) (void)hipFree(o.dmat[a]);
+    return r;
+  }
+  h->ops.push_back(o);
+  h->cur = (int)h->ops.size() - 1;
+  return CRYS_OK;
+}
+
+int crys_diffusion(crys_solver_t* h) {
+  CHK(check_handle(h));
+  return diffusion_impl(h);
+}
+
+int crys_get_cell_para(const crys_solver_t* h, double out[6]) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  memcpy(out, h->cell, sizeof(h->cell));
+  return CRYS_OK;
+}
+int crys_get_grid_size_log(const crys_solver_t* h, size_t out[3]) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  for (int a = 0; a < 3; ++a) out[a] = h->nlog[a];
+  return CRYS_OK;
+}
+int crys_get_grid_size_phy(const crys_solver_t* h, size_t out[3]) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  for (int a = 0; a < 3; ++a) out[a] = h->nphy[a];
+  return CRYS_OK;
+}
+int crys_get_space_size_log(const crys_solver_t* h, size_t* out) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  *out = h->n_log;
+  return CRYS_OK;
+}
+int crys_get_space_size_phy(const crys_solver_t* h, size_t* out) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  *out = h->n_phy;
+  return CRYS_OK;
+}
+int crys_get_ds(const crys_solver_t* h, double* out) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  *out = h->ds_current;
+  return CRYS_OK;
+}
+int crys_get_iomat_for_diffusion(const crys_solver_t* h, double** out) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  *out = h->iomat;
+  return CRYS_OK;
+}
+int crys_get_kind(const crys_solver_t* h, int* out) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  *out = h->kind;
+  return CRYS_OK;
+}
+int crys_get_precision(const crys_solver_t* h, int* out) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  *out = h->precision;
+  return CRYS_OK;
+}
+
+int crys_iomat_upload(crys_solver_t* h, const double* host) {
+  CHK(check_handle(h));
+  if (!host) return fail(CRYS_ERR_INVALID_ARGUMENT, "null host buffer");
+  HIPCHK(hipMemcpyAsync(h->iomat, host, h->n_phy * sizeof(double), hipMemcpyHostToDevice,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return CRYS_OK;
+}
+
+int crys_iomat_download(crys_solver_t* h, double* host) {
+  CHK(check_handle(h));
+  if (!host) return fail(CRYS_ERR_INVALID_ARGUMENT, "null host buffer");
+  HIPCHK(hipMemcpyAsync(host, h->iomat, h->n_phy * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return CRYS_OK;
+}
+
+int crys_solver_synchronize(crys_solver_t* h) {
+  CHK(check_handle(h));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return CRYS_OK;
+}
+
+int crys_solver_get_stream(const crys_solver_t* h, void** out) {
+  CHK(check_handle(h));
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  *out = (void*)h->stream;
+  return CRYS_OK;
+}
+
+int crys_propagate(crys_solver_t* h, const double* E, const double* init, size_t Ns,
+                   int direction, double* q) {
+  CHK(check_handle(h));
+  if (!E || !init || !q) return fail(CRYS_ERR_INVALID_ARGUMENT, "null device buffer");
+  if (direction != 1 && direction != -1)
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "Wrong direction");  // crysSCFT_sample.cpp:381
+  if (h->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
+  if (h->precision != CRYS_FP64)
+    return fail(CRYS_ERR_UNSUPPORTED, "crys_propagate takes FP64 histories; use an FP64 solver");
+  const size_t n = h->n_phy;
+  const OpSet& op = h->ops[h->cur];
+  size_t s = direction == 1 ? 0 : Ns;
+  HIPCHK(hipMemcpyAsync(q + s * n, init, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  double* scr = (double*)h->scratch;
+  for (size_t it = 0; it < Ns; ++it) {
+    const size_t sn = direction == 1 ? s + 1 : s - 1;
+    const double* in[2] = {q + s * n, nullptr};
+    double* out[2] = {q + sn * n, nullptr};
+    double* sc[2] = {scr, nullptr};
+    const double* Ee[2] = {E, nullptr};
+    CHK(launch_step<double>(h, op, 1, in, out, sc, Ee, h->stream));
+    s = sn;
+  }
+  HIPCHK(hipMemcpyAsync(h->iomat, q + s * n, n * sizeof(double), hipMemcpyDeviceToDevice,
+                        h->stream));
+  return CRYS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// SCFT iteration
+// ---------------------------------------------------------------------------
+namespace {
+
+void scft_free(crys_scft* h) {
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->exec = nullptr;
+  h->graph = nullptr;
+  void* bufs[] = {h->wA,    h->wB,    h->ksi,    h->phiA,   h->phiB,  h->EA,
+                  h->EB,    h->qf,    h->qb,     h->scr[0], h->scr[1], h->qpart,
+                  h->epart, h->scal,  h->trace,  h->counter};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+}
+
+// Records one full SCFT iteration (crysSCFT_sample.cpp:476-508) on stream s.
+// The forward chain is q_A then q_B (joined at slot NsA, :487), the backward
+// chain q_B^dagger then q_A^dagger (joined at slot NsA, :490); both advance
+// in the same launches.
+template <typename T>
+int scft_record_iteration(crys_scft* h, hipStream_t s) {
+  const crys_solver* sv = h->sv;
+  const OpSet& op = sv->ops[sv->cur];
+  const size_t n = h->n, Ns = h->Ns, NsA = h->NsA;
+  T* qf = (T*)h->qf;
+  T* qb = (T*)h->qb;
+  T* scr[2] = {(T*)h->scr[0], (T*)h->scr[1]};
+  const T* EA = (const T*)h->EA;
+  const T* EB = (const T*)h->EB;
+  for (size_t t = 0; t < Ns; ++t) {
+    const size_t bo = Ns - t - 1;
+    const T* in[2] = {qf + t * n, qb + (Ns - t) * n};
+    T* out[2] = {qf + (t + 1) * n, qb + bo * n};
+    const T* E[2] = {t < NsA ? EA : EB, bo >= NsA ? EB : EA};
+    CHK(launch_step<T>(sv, op, 2, in, out, scr, E, s));
+  }
+  const int nb = crys::red_blocks(n);
+  hipLaunchKernelGGL((crys::qsum_partial_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
+                     (const T*)qb, h->qpart, n);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(crys::q_finalize_kernel, dim3(1), dim3(crys::kRedThreads), 0, s,
+                     (const double*)h->qpart, nb, h->scal, n, h->fA, h->fB, h->NsA, h->NsB);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL((crys::phi_energy_mix_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
+                     (const T*)qf, (const T*)qb, (const double*)h->scal, h->phiA, h->phiB, h->wA,
+                     h->wB, h->ksi, (T*)h->EA, (T*)h->EB, h->epart, n, h->NsA, h->NsB, h->xN,
+                     h->acc, h->ds);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(crys::energy_finalize_kernel, dim3(1), dim3(crys::kRedThreads), 0, s,
+                     (const double*)h->epart, nb, (const double*)h->scal, h->trace, h->counter,
+                     kTraceCap, n);
+  HIPCHK(hipGetLastError());
+  return CRYS_OK;
+}
+
+int scft_build_graph(crys_scft* h) {
+  crys_solver* sv = h->sv;
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->exec = nullptr;
+  h->graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
+  int r = sv->precision == CRYS_FP32 ? scft_record_iteration<float>(h, sv->stream)
+                                     : scft_record_iteration<double>(h, sv->stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(sv->stream, &g);
+  if (r != CRYS_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return fail(CRYS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  h->graph = g;
+  HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  const OpSet& op = sv->ops[sv->cur];
+  for (int a = 0; a < 3; ++a) h->graph_mats[a] = op.dmat[a];
+  return CRYS_OK;
+}
+
+template <typename T>
+void scft_prep(crys_scft* h) {
+  hipLaunchKernelGGL((crys::prep_fields_kernel<T>), dim3(grid_for(h->n)), dim3(256), 0,
+                     h->sv->stream, (const double*)h->wA, (const double*)h->wB, h->ksi, (T*)h->EA,
+                     (T*)h->EB, h->xN, h->ds, h->n);
+}
+
+template <typename T>
+void scft_fill_ends(crys_scft* h) {
+  const size_t n = h->n;
+  hipLaunchKernelGGL((crys::fill_kernel<T>), dim3(grid_for(n)), dim3(256), 0, h->sv->stream,
+                     (T*)h->qf, (T)1, n);
+  hipLaunchKernelGGL((crys::fill_kernel<T>), dim3(grid_for(n)), dim3(256), 0, h->sv->stream,
+                     (T*)h->qb + h->Ns * n, (T)1, n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, double Lx, double Ly,
+                     double Lz, double acceptance, crys_scft_t** out) {
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  *out = nullptr;
+  CHK(check_handle(solver));
+  if (!(ds > 0) || !std::isfinite(ds)) return fail(CRYS_ERR_INVALID_ARGUMENT, "ds must be > 0");
+  if (!(fA > 0 && fA < 1)) return fail(CRYS_ERR_INVALID_ARGUMENT, "fA must lie in (0, 1)");
+  const double fB = 1 - fA;
+  const size_t NsA = (size_t)(fA / ds);  // crysSCFT_sample.cpp:443
+  const size_t NsB = (size_t)(fB / ds);  // :444
+  if (NsA == 0 || NsB == 0)
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "NsA = %zu, NsB = %zu: each block needs >= 1 step", NsA,
+                NsB);
+  const double cell[6] = {Lx, Ly, Lz, 0, 0, 0};
+  CHK(crys_set_cell_para(solver, cell));   // :466
+  CHK(crys_set_contour_step(solver, ds));  // :467
+  CHK(crys_diffusion(solver));             // :471 warm-up
+  crys_scft* h = new (std::nothrow) crys_scft();
+  if (!h) return fail(CRYS_ERR_OUT_OF_MEMORY, "host allocation failed");
+  h->sv = solver;
+  h->fA = fA;
+  h->fB = fB;
+  h->ds = ds;
+  h->xN = xN;
+  h->acc = acceptance;
+  h->L[0] = Lx;
+  h->L[1] = Ly;
+  h->L[2] = Lz;
+  h->NsA = NsA;
+  h->NsB = NsB;
+  h->Ns = NsA + NsB;
+  h->n = solver->n_phy;
+  const size_t n = h->n, es = elem_size(solver->precision);
+  hipError_t e = hipSuccess;
+  auto A = [&](void** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+  };
+  A((void**)&h->wA, n * 8);
+  A((void**)&h->wB, n * 8);
+  A((void**)&h->ksi, n * 8);
+  A((void**)&h->phiA, n * 8);
+  A((void**)&h->phiB, n * 8);
+  A(&h->EA, n * es);
+  A(&h->EB, n * es);
+  A(&h->qf, (h->Ns + 1) * n * es);
+  A(&h->qb, (h->Ns + 1) * n * es);
+  A(&h->scr[0], n * es);
+  A(&h->scr[1], n * es);
+  A((void**)&h->qpart, crys::kMaxRedBlocks * 8);
+  A((void**)&h->epart, 4 * crys::kMaxRedBlocks * 8);
+  A((void**)&h->scal, 4 * 8);
+  A((void**)&h->trace, (size_t)kTraceCap * CRYS_TRACE_COLS * 8);
+  A((void**)&h->counter, sizeof(int));
+  if (e != hipSuccess) {
+    scft_free(h);
+    delete h;
+    return fail(e == hipErrorOutOfMemory ? CRYS_ERR_OUT_OF_MEMORY : CRYS_ERR_HIP,
+                "SCFT allocation failed: %s", hipGetErrorString(e));
+  }
+  if (solver->precision == CRYS_FP32) scft_fill_ends<float>(h);
+  else scft_fill_ends<double>(h);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(solver->stream);
+  if (e != hipSuccess) {
+    scft_free(h);
+    delete h;
+    return fail(CRYS_ERR_HIP, "SCFT setup failed: %s", hipGetErrorString(e));
+  }
+  *out = h;
+  return CRYS_OK;
+}
+
+int crys_scft_destroy(crys_scft_t* h) {
+  if (!h) return CRYS_OK;
+  if (h->sv && h->sv->stream) (void)hipStreamSynchronize(h->sv->stream);
+  scft_free(h);
+  delete h;
+  return CRYS_OK;
+}
+
+int crys_scft_set_fields(crys_scft_t* h, const double* wA, const double* wB) {
+  if (!h || !wA || !wB) return fail(CRYS_ERR_INVALID_ARGUMENT, "null argument");
+  hipStream_t s = h->sv->stream;
+  HIPCHK(hipMemcpyAsync(h->wA, wA, h->n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(h->wB, wB, h->n * 8, hipMemcpyHostToDevice, s));
+  if (h->sv->precision == CRYS_FP32) scft_prep<float>(h);
+  else scft_prep<double>(h);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  h->fields_set = true;
+  return CRYS_OK;
+}
+
+int crys_scft_run(crys_scft_t* h, size_t nstep, double* trace_host) {
+  if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
+  if (!h->fields_set) return fail(CRYS_ERR_INVALID_ARGUMENT, "crys_scft_set_fields not called");
+  crys_solver* sv = h->sv;
+  if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
+  const OpSet& op = sv->ops[sv->cur];
+  bool stale = !h->exec;
+  for (int a = 0; a < 3; ++a) stale = stale || h->graph_mats[a] != op.dmat[a];
+  if (stale) CHK(scft_build_graph(h));
+  hipStream_t s = sv->stream;
+  bool diverged = false;
+  size_t done = 0;
+  std::vector<double> rows;
+  while (done < nstep) {
+    const size_t chunk = nstep - done < (size_t)kTraceCap ? nstep - done : (size_t)kTraceCap;
+    hipLaunchKernelGGL(crys::set_int_kernel, dim3(1), dim3(1), 0, s, h->counter, 0);
+    HIPCHK(hipGetLastError());
+    for (size_t i = 0; i < chunk; ++i) HIPCHK(hipGraphLaunch(h->exec, s));
+    if (trace_host) {
+      HIPCHK(hipMemcpyAsync(trace_host + done * CRYS_TRACE_COLS, h->trace,
+                            chunk * CRYS_TRACE_COLS * 8, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (trace_host)
+      for (size_t i = 0; i < chunk; ++i) {
+        const double* r = trace_host + (done + i) * CRYS_TRACE_COLS;
+        if (!std::isfinite(r[CRYS_TRACE_F]) || !std::isfinite(r[CRYS_TRACE_Q])) diverged = true;
+      }
+    done += chunk;
+  }
+  if (diverged) return fail(CRYS_ERR_DIVERGED, "non-finite free energy or Q in the trace");
+  return CRYS_OK;
+}
+
+int crys_scft_get_fields(crys_scft_t* h, double* phiA, double* phiB, double* wA, double* wB) {
+  if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
+  hipStream_t s = h->sv->stream;
+  const size_t b = h->n * 8;
+  if (phiA) HIPCHK(hipMemcpyAsync(phiA, h->phiA, b, hipMemcpyDeviceToHost, s));
+  if (phiB) HIPCHK(hipMemcpyAsync(phiB, h->phiB, b, hipMemcpyDeviceToHost, s));
+  if (wA) HIPCHK(hipMemcpyAsync(wA, h->wA, b, hipMemcpyDeviceToHost, s));
+  if (wB) HIPCHK(hipMemcpyAsync(wB, h->wB, b, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return CRYS_OK;
+}
+
+int crys_scft_get_contour(const crys_scft_t* h, size_t* NsA, size_t* NsB) {
+  if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
+  if (NsA) *NsA = h->NsA;
+  if (NsB) *NsB = h->NsB;
+  return CRYS_OK;
+}
+
+int crys_diblock_scft_simple_iterate(crys_solver_t* solver, double fA, double ds, double xN,
+                                     double Lx, double Ly, double Lz, double acceptance,
+                                     size_t nstep, const double* wA_init, const double* wB_init,
+                                     double* trace, double* phiA, double* phiB, double* wA_out,
+                                     double* wB_out) {
+  crys_scft* h = nullptr;
+  CHK(crys_scft_create(solver, fA, ds, xN, Lx, Ly, Lz, acceptance, &h));
+  int r = crys_scft_set_fields(h, wA_init, wB_init);
+  if (r == CRYS_OK) r = crys_scft_run(h, nstep, trace);
+  if (r == CRYS_OK || r == CRYS_ERR_DIVERGED) {
+    int r2 = crys_scft_get_fields(h, phiA, phiB, wA_out, wB_out);
+    if (r == CRYS_OK) r = r2;
+  }
+  std::string keep = g_err;
+  crys_scft_destroy(h);
+  g_err = keep;
+  return r;
+}
+
+int crys_axis_operator(int kind, size_t n_logical, double L, double ds, double* out) {
+  if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");
+  if (!(ds > 0) || !(L > 0)) return fail(CRYS_ERR_INVALID_ARGUMENT, "L and ds must be > 0");
+  if (n_logical == 0 || n_logical > 256)
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "n_logical out of range");
+  std::vector<double> At;
+  if (kind == CRYS_P1) At = crys::p1_axis_operator((int)n_logical, L, ds);
+  else if (kind == CRYS_3M || kind == CRYS_PMMM_DCT) {
+    if (n_logical % 2) return fail(CRYS_ERR_BAD_GRID, "n_logical must be even");
+    At = kind == CRYS_3M ? crys::m3_axis_operator((int)n_logical, L, ds)
+                         : crys::pmmm_axis_operator((int)n_logical / 2, L, ds);
+  } else
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "unknown solver kind %d", kind);
+  memcpy(out, At.data(), At.size() * sizeof(double));
+  return CRYS_OK;
+}
+
+int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* ms_per_pass) {
+  if (!h || reps <= 0) return fail(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
+  crys_solver* sv = h->sv;
+  if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
+  const OpSet& op = sv->ops[sv->cur];
+  hipStream_t s = sv->stream;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const size_t n = h->n, Ns = h->Ns;
+  int r = CRYS_OK;
+  auto step = [&]() -> int {
+    if (sv->precision == CRYS_FP32) {
+      float* qf = (float*)h->qf;
+      float* qb = (float*)h->qb;
+      const float* in[2] = {qf, qb + Ns * n};
+      float* out[2] = {qf + n, qb + (Ns - 1) * n};
+      float* sc[2] = {(float*)h->scr[0], (float*)h->scr[1]};
+      const float* E[2] = {(const float*)h->EA, (const float*)h->EB};
+      return launch_step<float>(sv, op, 2, in, out, sc, E, s);
+    }
+    double* qf = (double*)h->qf;
+    double* qb = (double*)h->qb;
+    const double* in[2] = {qf, qb + Ns * n};
+    double* out[2] = {qf + n, qb + (Ns - 1) * n};
+    double* sc[2] = {(double*)h->scr[0], (double*)h->scr[1]};
+    const double* E[2] = {(const double*)h->EA, (const double*)h->EB};
+    return launch_step<double>(sv, op, 2, in, out, sc, E, s);
+  };
+  r = step();  // warm
+  if (r == CRYS_OK) {
+    (void)hipEventRecord(e0, s);
+    for (int i = 0; i < reps && r == CRYS_OK; ++i) r = step();
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms_per_step) *ms_per_step = ms / reps;
+    if (ms_per_pass) *ms_per_pass = ms / (3.0 * reps);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return r;
+}
+
+}  // extern "C"

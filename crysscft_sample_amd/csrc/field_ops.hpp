@@ -1,0 +1,200 @@
+// Pointwise and reduction kernels of the SCFT iteration
+// (crysSCFT_sample.cpp:294-371, :398-424, :476-508) on the device.
+//
+// Reductions are deterministic: a fixed grid, a fixed per-thread order and a
+// fixed LDS tree, then a single-workgroup pass over the per-block partials.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace crys {
+
+constexpr int kRedThreads = 256;
+constexpr int kMaxRedBlocks = 512;
+
+inline int red_blocks(size_t n) {
+  size_t b = (n + kRedThreads - 1) / kRedThreads;
+  return (int)(b < (size_t)kMaxRedBlocks ? (b ? b : 1) : kMaxRedBlocks);
+}
+
+template <typename T>
+__global__ void fill_kernel(T* p, T v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <typename D, typename S>
+__global__ void convert_kernel(D* dst, const S* src, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (D)src[i];
+}
+
+// calc_ksi (:341-346) and array_exp (:315-322) for both blocks.
+template <typename T>
+__global__ void prep_fields_kernel(const double* wA, const double* wB, double* ksi, T* EA, T* EB,
+                                   double xN, double ds, size_t n) {
+  const double index = -ds / 2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double a = wA[i], b = wB[i];
+    ksi[i] = 0.5 * (a + b - xN);
+    EA[i] = (T)(1. * exp(index * a));
+    EB[i] = (T)(1. * exp(index * b));
+  }
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double block_max(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] > sh[t + s] ? sh[t] : sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Partial sums of q(s=0) of the backward chain for Q = <q_A^dagger(0)>
+// (array_average, :324-331, called at :494).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void qsum_partial_kernel(const T* q0, double* part,
+                                                                   size_t n) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kRedThreads)
+    acc += (double)q0[i];
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// scal[0] = Q = sum(part)/n, scal[1] = fA/(NsA Q), scal[2] = fB/(NsB Q).
+__global__ __launch_bounds__(kRedThreads) void q_finalize_kernel(const double* part, int nparts,
+                                                                 double* scal, size_t n, double fA,
+                                                                 double fB, size_t NsA,
+                                                                 size_t NsB) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kRedThreads) acc += part[i];
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) {
+    const double Q = s / n;
+    scal[0] = Q;
+    scal[1] = fA / (NsA * Q);
+    scal[2] = fB / (NsB * Q);
+  }
+}
+
+// integrate_phi (:348-360) for both blocks from the joined histories
+// (forward slots 0..NsA are q_A, NsA..NsA+NsB are q_B; backward slots the
+// same for q^dagger), then print_energy's sums (:404-418) with the
+// pre-update fields, calc_new_fields (:362-371), and the next iteration's
+// calc_ksi / array_exp (:478-480).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
+    const T* qf, const T* qb, const double* scal, double* phiA, double* phiB, double* wA,
+    double* wB, double* ksi, T* EA, T* EB, double* part, size_t n, size_t NsA, size_t NsB,
+    double xN, double acc_mix, double ds) {
+  __shared__ double sh[kRedThreads];
+  const double ca = scal[1], cb = scal[2];
+  const double index = -ds / 2;
+  double s_e1 = 0.0, s_enth = 0.0, s_fk = 0.0, m_inc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kRedThreads) {
+    double pA = 0.0;
+    for (size_t s = 0; s <= NsA; ++s) {
+      double factor = s == 0 ? 0.5 : (s == NsA ? 0.5 : 1);
+      factor *= ca;
+      pA += factor * (double)qf[s * n + i] * (double)qb[s * n + i];
+    }
+    double pB = 0.0;
+    for (size_t s = 0; s <= NsB; ++s) {
+      double factor = s == 0 ? 0.5 : (s == NsB ? 0.5 : 1);
+      factor *= cb;
+      const size_t k = (NsA + s) * n + i;
+      pB += factor * (double)qf[k] * (double)qb[k];
+    }
+    const double a = wA[i], b = wB[i], x = ksi[i];
+    s_e1 += -a * pA - b * pB;
+    s_enth += xN * pA * pB;
+    const double t = 1 - pA - pB;
+    s_fk += -x * t;
+    m_inc = fabs(t) > m_inc ? fabs(t) : m_inc;
+    phiA[i] = pA;
+    phiB[i] = pB;
+    const double na = a + acc_mix * (xN * pB + x - a);
+    const double nb = b + acc_mix * (xN * pA + x - b);
+    wA[i] = na;
+    wB[i] = nb;
+    ksi[i] = 0.5 * (na + nb - xN);
+    EA[i] = (T)(1. * exp(index * na));
+    EB[i] = (T)(1. * exp(index * nb));
+  }
+  const double r0 = block_sum(s_e1, sh);
+  const double r1 = block_sum(s_enth, sh);
+  const double r2 = block_sum(s_fk, sh);
+  const double r3 = block_max(m_inc, sh);
+  if (threadIdx.x == 0) {
+    part[0 * kMaxRedBlocks + blockIdx.x] = r0;
+    part[1 * kMaxRedBlocks + blockIdx.x] = r1;
+    part[2 * kMaxRedBlocks + blockIdx.x] = r2;
+    part[3 * kMaxRedBlocks + blockIdx.x] = r3;
+  }
+}
+
+// Sums the partials and writes one trace row at row *counter (mod cap).
+__global__ __launch_bounds__(kRedThreads) void energy_finalize_kernel(const double* part,
+                                                                      int nparts,
+                                                                      const double* scal,
+                                                                      double* trace, int* counter,
+                                                                      int cap, size_t n) {
+  __shared__ double sh[kRedThreads];
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int i = threadIdx.x; i < nparts; i += kRedThreads) {
+    a0 += part[0 * kMaxRedBlocks + i];
+    a1 += part[1 * kMaxRedBlocks + i];
+    a2 += part[2 * kMaxRedBlocks + i];
+    const double m = part[3 * kMaxRedBlocks + i];
+    a3 = m > a3 ? m : a3;
+  }
+  const double e1 = block_sum(a0, sh);
+  const double enth = block_sum(a1, sh);
+  const double fk = block_sum(a2, sh);
+  const double mi = block_max(a3, sh);
+  if (threadIdx.x == 0) {
+    const double Q = scal[0];
+    const double e0 = -log(Q);
+    const double E1 = e1 / n, EN = enth / n, FK = fk / n;
+    const int row = *counter % cap;
+    double* r = trace + (size_t)row * 7;
+    r[0] = e0 + E1 + EN + FK;
+    r[1] = EN;
+    r[2] = e0;
+    r[3] = E1;
+    r[4] = FK;
+    r[5] = mi;
+    r[6] = Q;
+    *counter = *counter + 1;
+  }
+}
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+}  // namespace crys

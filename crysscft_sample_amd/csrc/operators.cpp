@@ -1,0 +1,88 @@
+// Host construction of the per-axis diffusion operators (see operators.hpp).
+// Sums run in long double with exact integer range reduction of the cosine
+// arguments, so each entry is the double nearest the exact operator to within
+// a few ulp.
+#include "operators.hpp"
+
+#include <cmath>
+
+namespace crys {
+
+namespace {
+
+constexpr long double kPiL = 3.141592653589793238462643383279502884L;
+
+// cos(2 pi m / d) for m in [0, d)
+std::vector<long double> cos_table(long d) {
+  std::vector<long double> t(d);
+  for (long m = 0; m < d; ++m) t[m] = cosl(2.0L * kPiL * (long double)m / (long double)d);
+  return t;
+}
+
+// exp(-ds (2 pi k / L)^2), the per-axis factor of K.
+long double kfactor(long k, double L, double ds) {
+  long double kk = 2.0L * kPiL * (long double)k / (long double)L;
+  return expl(-(long double)ds * kk * kk);
+}
+
+std::vector<long double> p1_dense(int N, double L, double ds) {
+  std::vector<long double> K(N);
+  for (int k = 0; k < N; ++k) K[k] = kfactor(k < (N + 1) / 2 ? k : N - k, L, ds);
+  std::vector<long double> ct = cos_table(N);
+  // circulant: D[j][j'] = c(j - j') with c(d) = (1/N) sum_k K(k) cos(2 pi k d / N)
+  std::vector<long double> c(N);
+  for (int d = 0; d < N; ++d) {
+    long double s = 0;
+    for (int k = 0; k < N; ++k) s += K[k] * ct[((long)k * d) % N];
+    c[d] = s / (long double)N;
+  }
+  std::vector<long double> D((size_t)N * N);
+  for (int j = 0; j < N; ++j)
+    for (int jp = 0; jp < N; ++jp) D[(size_t)j * N + jp] = c[((j - jp) % N + N) % N];
+  return D;
+}
+
+}  // namespace
+
+std::vector<double> pmmm_axis_operator(int M, double L, double ds) {
+  // cos(pi k (2j+1) / (2M)) = cos(2 pi m / (4M)), m = k (2j+1) mod 4M
+  std::vector<long double> ct = cos_table(4L * M);
+  std::vector<long double> K(M);
+  for (int k = 0; k < M; ++k) K[k] = kfactor(k, L, ds);
+  std::vector<double> At((size_t)M * M);
+  for (int j = 0; j < M; ++j)
+    for (int jp = 0; jp < M; ++jp) {
+      long double s = 0;
+      for (int k = 0; k < M; ++k) {
+        long double ck = k == 0 ? 1.0L : 2.0L;
+        long double a = ct[((long)k * (2 * j + 1)) % (4L * M)];
+        long double b = ct[((long)k * (2 * jp + 1)) % (4L * M)];
+        s += ck * a * K[k] * 2.0L * b;
+      }
+      // D[j][jp]; D is symmetric, stored transposed all the same
+      At[(size_t)jp * M + j] = (double)(s / (2.0L * M));
+    }
+  return At;
+}
+
+std::vector<double> p1_axis_operator(int N, double L, double ds) {
+  std::vector<long double> D = p1_dense(N, L, ds);
+  std::vector<double> At((size_t)N * N);
+  for (int j = 0; j < N; ++j)
+    for (int i = 0; i < N; ++i) At[(size_t)i * N + j] = (double)D[(size_t)j * N + i];
+  return At;
+}
+
+std::vector<double> m3_axis_operator(int N, double L, double ds) {
+  const int M = N / 2;
+  std::vector<long double> D = p1_dense(N, L, ds);
+  std::vector<double> At((size_t)M * M);
+  for (int j = 0; j < M; ++j)
+    for (int i = 0; i < M; ++i) {
+      long double v = D[(size_t)(2 * j) * N + 2 * i] + D[(size_t)(2 * j) * N + (2 * M - 1 - 2 * i)];
+      At[(size_t)i * M + j] = (double)v;
+    }
+  return At;
+}
+
+}  // namespace crys

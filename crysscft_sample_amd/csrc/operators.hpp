@@ -1,0 +1,33 @@
+// Per-axis diffusion operators of the three solvers, built on the host.
+//
+// In every solver the Fourier multiplier K = exp(-ds (kx^2 + ky^2 + kz^2))
+// factorises over the axes, and so do the transforms (FFTW r2c/c2r for P1,
+// REDFT10/REDFT01 for Pmmm).  The whole triplet "forward transform, xK,
+// inverse transform, normalise" is therefore the tensor product
+// Dx (x) Dy (x) Dz of three small real matrices, one per axis.  This file
+// builds those matrices in extended precision; the device applies them with
+// one pass per axis.
+//
+//  - PmmmDCT (solver_Pmmm.h:125-215, K from :159-186):
+//      D = C3 diag(K) C2 / (2M), C2[k][j] = 2 cos(pi k (j+1/2)/M) (REDFT10),
+//      C3[j][k] = c_k cos(pi k (j+1/2)/M), c_0 = 1, c_k = 2 (REDFT01).
+//  - P1 (solver_p1.h:83-202, K from :142-172):
+//      D[j][j'] = (1/N) sum_k K(k) cos(2 pi k (j-j')/N), k folded as
+//      k < (N+1)/2 ? k : N-k.
+//  - 3m with zero translational parts (solver_3m.h:143-324): the even
+//      sub-lattice e[j] = f[2j] of a field with f(N-1-j) = f(j) fixes f on
+//      the odd sites (f[2j+1] = e[M-1-j]), so D3[j][j'] = Dp1[2j][2j'] +
+//      Dp1[2j][2M-1-2j'] with the P1 operator of the logical length N = 2M.
+#pragma once
+
+#include <vector>
+
+namespace crys {
+
+// Each builder returns At with At[i * M + j] = D[j][i], i.e. out_j =
+// sum_i At[i*M + j] * in_i (the layout the device pass streams).
+std::vector<double> pmmm_axis_operator(int M, double L, double ds);
+std::vector<double> p1_axis_operator(int N, double L, double ds);
+std::vector<double> m3_axis_operator(int N_logical, double L, double ds);
+
+}  // namespace crys

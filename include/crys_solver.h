@@ -1,0 +1,192 @@
+/*
+ * crys_solver.h -- C ABI of the MI355X (gfx950) propagator library
+ * libcrys_solver.so.
+ *
+ * The library is a drop-in for the reference's solver plugin API
+ * (CrysPSSolverBase, solver_base.h:8-51, and its three implementations
+ * CrysPSSolverP1 / CrysPSSolver3m / CrysPSSolverPmmm) and for the SCFT
+ * iteration that calls it (diblock_scft_simple_iterate,
+ * crysSCFT_sample.cpp:426-531).  All buffers the handle owns live in device
+ * memory (HBM); host staging is explicit.  Every call returns a status code
+ * and never exits the process; crys_last_error() describes the last failure
+ * on the calling thread.
+ *
+ * Threading: calls on one handle are serialised by the caller; different
+ * handles may be used concurrently from different threads (each handle owns
+ * its own HIP stream).
+ */
+#ifndef CRYS_SOLVER_H
+#define CRYS_SOLVER_H
+
+#include <stddef.h>
+
+#if defined(__GNUC__)
+#define CRYS_API __attribute__((visibility("default")))
+#else
+#define CRYS_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct crys_solver crys_solver_t;
+typedef struct crys_scft crys_scft_t;
+
+/* Solver kinds: CrysPSSolverP1 (solver_p1.h:14), CrysPSSolver3m
+ * (solver_3m.h:14), CrysPSSolverPmmm "SolverPmmmDCT" (solver_Pmmm.h:13). */
+enum crys_kind { CRYS_P1 = 0, CRYS_3M = 1, CRYS_PMMM_DCT = 2 };
+
+/* Arithmetic of the propagators.  CRYS_FP64 is the plain method. */
+enum crys_precision { CRYS_FP64 = 0, CRYS_FP32 = 1 };
+
+/* Creation flags. */
+enum crys_flags {
+  CRYS_FLAG_NONE = 0,
+  /* PmmmDCT only: honour Lx, Ly, Lz.  Without it the solver reproduces the
+   * reference, which keeps only cell_para[0] (solver_Pmmm.h:101). */
+  CRYS_FLAG_PMMM_ORTHORHOMBIC = 1
+};
+
+enum crys_status {
+  CRYS_OK = 0,
+  CRYS_ERR_INVALID_ARGUMENT = 1, /* NULL pointer, bad direction, ds <= 0, ... */
+  CRYS_ERR_BAD_GRID = 2,         /* the grid the reference's set_grid() rejects */
+  CRYS_ERR_NO_CONTOUR_STEP = 3,  /* diffusion before set_contour_step */
+  CRYS_ERR_UNSUPPORTED = 4,      /* a combination this build does not run */
+  CRYS_ERR_HIP = 5,              /* a HIP runtime call failed */
+  CRYS_ERR_OUT_OF_MEMORY = 6,
+  CRYS_ERR_DIVERGED = 7          /* non-finite free energy or Q */
+};
+
+/* Description of the last error on this thread ("" if none). */
+CRYS_API const char* crys_last_error(void);
+/* Library version string. */
+CRYS_API const char* crys_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+CRYS_API int crys_device_count(void);
+/* hipDeviceSynchronize on the current device. */
+CRYS_API int crys_device_synchronize(void);
+
+/* ---- solver plugin API ------------------------------------------------ */
+
+/* Constructors CrysPSSolverP1(grid) (solver_p1.h:16),
+ * CrysPSSolver3m(grid, translational_part) (solver_3m.h:17) and
+ * CrysPSSolverPmmm(grid) (solver_Pmmm.h:16), with the grid checks of their
+ * set_grid() (solver_p1.h:95-125, solver_3m.h:73-113, solver_Pmmm.h:66-96)
+ * returning CRYS_ERR_BAD_GRID instead of exit(1).
+ * grid_log: logical grid {Nx, Ny, Nz}.  translational_part: 9 doubles for
+ * CRYS_3M (glide parts of the -z, -y, -x planes), may be NULL (all zero);
+ * ignored for other kinds.  precision: enum crys_precision.  flags: enum
+ * crys_flags. */
+CRYS_API int crys_solver_create(int kind, const size_t grid_log[3], const double translational_part[9],
+                       int precision, int flags, crys_solver_t** out);
+/* ~CrysPSSolver* (solver_p1.h:55-62, solver_3m.h:55-71, solver_Pmmm.h:54-64). */
+CRYS_API int crys_solver_destroy(crys_solver_t* h);
+
+/* set_cell_para (solver_base.h:16): {Lx, Ly, Lz, alpha, beta, gamma}; the
+ * angles are reserved, as in the reference.  Early-return and cubic-only
+ * rules of solver_3m.h:115-121 / solver_Pmmm.h:98-104 are kept; P1 always
+ * resets (solver_p1.h:64-69).  A change of cell drops the cached operators
+ * and resets ds to 0. */
+CRYS_API int crys_set_cell_para(crys_solver_t* h, const double cell_para[6]);
+/* set_contour_step (solver_base.h:18): operators are cached per ds value,
+ * like the reference's std::map (solver_p1.h:71-81, solver_Pmmm.h:106-123). */
+CRYS_API int crys_set_contour_step(crys_solver_t* h, double ds);
+/* diffusion() (solver_base.h:20): iomat <- T^-1 K T iomat, normalisation
+ * included, on the device-resident iomat; asynchronous on the handle's
+ * stream.  E = exp(-w ds/2) is applied by the caller, as in the reference. */
+CRYS_API int crys_diffusion(crys_solver_t* h);
+
+/* Getters (solver_base.h:22-28). */
+CRYS_API int crys_get_cell_para(const crys_solver_t* h, double out[6]);
+CRYS_API int crys_get_grid_size_log(const crys_solver_t* h, size_t out[3]);
+CRYS_API int crys_get_grid_size_phy(const crys_solver_t* h, size_t out[3]);
+CRYS_API int crys_get_space_size_log(const crys_solver_t* h, size_t* out);
+CRYS_API int crys_get_space_size_phy(const crys_solver_t* h, size_t* out);
+CRYS_API int crys_get_ds(const crys_solver_t* h, double* out);
+/* get_iomat_for_diffusion (solver_base.h:28): a DEVICE pointer to
+ * space_size_phy doubles, owned by the handle. */
+CRYS_API int crys_get_iomat_for_diffusion(const crys_solver_t* h, double** out_device_ptr);
+CRYS_API int crys_get_kind(const crys_solver_t* h, int* out);
+CRYS_API int crys_get_precision(const crys_solver_t* h, int* out);
+
+/* Explicit host staging of iomat (space_size_phy doubles); synchronous. */
+CRYS_API int crys_iomat_upload(crys_solver_t* h, const double* host);
+CRYS_API int crys_iomat_download(crys_solver_t* h, double* host);
+/* Wait for the handle's stream. */
+CRYS_API int crys_solver_synchronize(crys_solver_t* h);
+/* The handle's HIP stream (hipStream_t), for callers that order work. */
+CRYS_API int crys_solver_get_stream(const crys_solver_t* h, void** out_stream);
+
+/* solve_diffusion_equation (crysSCFT_sample.cpp:373-396) on device buffers:
+ * q_hist[s0] = init; then Ns times: iomat *= E; diffusion(); iomat *= E;
+ * q_hist[s0 + direction] = iomat.  s0 = 0 for direction +1 and Ns for -1.
+ * E_dev, init_dev: space_size_phy doubles; q_hist_dev: (Ns+1) of them.
+ * On return iomat holds the last state.  Asynchronous on the handle's stream. */
+CRYS_API int crys_propagate(crys_solver_t* h, const double* E_dev, const double* init_dev, size_t Ns,
+                   int direction, double* q_hist_dev);
+
+/* ---- SCFT iteration (diblock_scft_simple_iterate) ---------------------- */
+
+/* Columns of one trace row, one row per iteration (print_energy,
+ * crysSCFT_sample.cpp:398-424, recorded every iteration at full precision). */
+enum crys_trace_col {
+  CRYS_TRACE_F = 0,          /* FreeEnergy */
+  CRYS_TRACE_ENTHALPIC = 1,  /* <xN phiA phiB> */
+  CRYS_TRACE_MINUS_LNQ = 2,  /* -ln Q */
+  CRYS_TRACE_ENTROPIC = 3,   /* <-wA phiA - wB phiB> */
+  CRYS_TRACE_KSI = 4,        /* <-ksi (1 - phiA - phiB)> */
+  CRYS_TRACE_MAXINCOMP = 5,  /* max |1 - phiA - phiB| */
+  CRYS_TRACE_Q = 6,          /* single-chain partition function */
+  CRYS_TRACE_COLS = 7
+};
+
+/* Sets up an AB-diblock run on `solver` (not owned): NsA = (size_t)(fA/ds),
+ * NsB = (size_t)((1-fA)/ds) (crysSCFT_sample.cpp:443-444), calls
+ * set_cell_para({Lx,Ly,Lz}) and set_contour_step(ds) (:466-467) and the
+ * warm-up diffusion (:471), and allocates the device fields and the
+ * (NsA+NsB+1)-slot forward and backward propagator histories. */
+CRYS_API int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, double Lx, double Ly,
+                     double Lz, double acceptance, crys_scft_t** out);
+CRYS_API int crys_scft_destroy(crys_scft_t* h);
+/* Initial fields in the solver's physical layout (init_func, :463). */
+CRYS_API int crys_scft_set_fields(crys_scft_t* h, const double* wA_host, const double* wB_host);
+/* Runs nstep iterations of :476-508 on the device.  trace_host (may be NULL)
+ * receives nstep rows of CRYS_TRACE_COLS doubles.  Synchronous. */
+CRYS_API int crys_scft_run(crys_scft_t* h, size_t nstep, double* trace_host);
+/* Fields after the last iteration (any pointer may be NULL).  phi are the
+ * last iteration's densities, w the fields after its mixing update. */
+CRYS_API int crys_scft_get_fields(crys_scft_t* h, double* phiA, double* phiB, double* wA, double* wB);
+CRYS_API int crys_scft_get_contour(const crys_scft_t* h, size_t* NsA, size_t* NsB);
+
+/* One-shot mirror of diblock_scft_simple_iterate (crysSCFT_sample.cpp:426-531)
+ * with the fields passed explicitly instead of through init_func.  Output
+ * pointers may be NULL. */
+CRYS_API int crys_diblock_scft_simple_iterate(crys_solver_t* solver, double fA, double ds, double xN,
+                                     double Lx, double Ly, double Lz, double acceptance,
+                                     size_t nstep, const double* wA_init, const double* wB_init,
+                                     double* trace, double* phiA, double* phiB, double* wA_out,
+                                     double* wB_out);
+
+/* ---- introspection (host only, no device needed) ------------------------ */
+
+/* The per-axis operator the device applies for one axis of a solver: D of
+ * length m (m = n_logical for CRYS_P1, n_logical/2 otherwise), written as
+ * out[i*m + j] = D[j][i] (m*m doubles).  The diffusion() of the solver is
+ * Dx (x) Dy (x) Dz; this exposes the factor that replaces the reference's
+ * transform/K/inverse-transform triplet (solver_p1.h:184-202,
+ * solver_3m.h:281-324, solver_Pmmm.h:194-215) for checks on the host. */
+CRYS_API int crys_axis_operator(int kind, size_t n_logical, double L, double ds, double* out);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* Times `reps` back-to-back launches of one propagation step (both chains,
+ * every pass of one contour step) of an SCFT handle on its stream with HIP
+ * events; writes the mean milliseconds per step and per pass launch. */
+CRYS_API int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* ms_per_pass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRYS_SOLVER_H */
