@@ -52,53 +52,86 @@ struct PassShape {
 template <typename T, int AXIS, int MP>
 __global__ __launch_bounds__(256) void axis_pass_kernel(PassParams<T> p) {
   using S = PassShape<MP>;
+  constexpr int NT = 256;
   constexpr int RJ = S::RJ, RT = S::RT, TL = S::TL, LDT = S::LDT;
+  constexpr int KT = MP * TL / NT;             // tile elements per thread
+  constexpr int KM = (MP * MP + NT - 1) / NT;  // operator elements per thread
+  static_assert(MP * TL % NT == 0, "tile must split evenly over the workgroup");
   __shared__ T tile[MP * LDT];
   __shared__ T mat[S::MAT_LDS ? MP * MP : 1];
 
   const PassChain<T> ch = p.c[blockIdx.y];
   const int X = p.X, Y = p.Y, Z = p.Z;
   const int M = AXIS == 0 ? X : (AXIS == 1 ? Y : Z);
-  const long YZ = (long)Y * Z;
-  const long nlines = (long)X * YZ / M;
-  const long l0 = (long)blockIdx.x * TL;
-  const int nl = (int)((nlines - l0) < TL ? (nlines - l0) : TL);  // valid lines in this tile
+  const int YZ = Y * Z;
+  const int nlines = X * YZ / M;
+  const int l0 = blockIdx.x * TL;
+  const int nl = nlines - l0 < TL ? nlines - l0 : TL;  // valid lines in this tile
   const int tid = threadIdx.x;
+  // a y-pass tile stays inside one x plane when TL divides Z
+  const bool yplane = AXIS == 1 && Z % TL == 0;
+  const int x0 = yplane ? l0 / Z : 0;
+  const int z0 = yplane ? l0 - x0 * Z : 0;
 
+  // ---- element (row i, line ll) handled by slot k of this thread ----
+  int gi[KT], lo[KT];
+  bool ok[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int f = tid + NT * k;
+    int i, ll;
+    if constexpr (AXIS == 2) {
+      ll = f / MP;
+      i = f - ll * MP;
+    } else {
+      i = f / TL;
+      ll = f - i * TL;
+    }
+    ok[k] = i < M && ll < nl;
+    lo[k] = i * LDT + ll;
+    int g;
+    if constexpr (AXIS == 2) {
+      g = (l0 + ll) * Z + i;
+    } else if constexpr (AXIS == 1) {
+      if (yplane) {
+        g = x0 * YZ + i * Z + z0 + ll;
+      } else {
+        const int l = l0 + ll, x = l / Z;
+        g = x * YZ + i * Z + (l - x * Z);
+      }
+    } else {
+      g = i * YZ + l0 + ll;
+    }
+    gi[k] = ok[k] ? g : 0;
+  }
+
+  // ---- issue every global load first, then fill LDS ----
+  T v[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) v[k] = ok[k] ? ch.in[gi[k]] : T(0);
+  if (ch.pre) {
+    T e[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.pre[gi[k]] : T(0);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] *= e[k];
+  }
   if constexpr (S::MAT_LDS) {
-    for (int f = tid; f < MP * MP; f += 256) {
-      const int i = f / MP, j = f - i * MP;
-      mat[f] = (i < M && j < M) ? p.mat[i * M + j] : T(0);
+    T m[KM];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const int f = tid + NT * k;
+      const int i = f / MP, j = f - (f / MP) * MP;
+      m[k] = (f < MP * MP && i < M && j < M) ? p.mat[i * M + j] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const int f = tid + NT * k;
+      if (f < MP * MP) mat[f] = m[k];
     }
   }
-
-  // ---- stage the M x TL tile (zero for lines past the end) ----
-  if constexpr (AXIS == 2) {
-    const long base = l0 * Z;
-    for (int f = tid; f < TL * M; f += 256) {
-      const int ll = f / M, i = f - ll * M;
-      T v = T(0);
-      if (ll < nl) {
-        v = ch.in[base + f];
-        if (ch.pre) v *= ch.pre[base + f];
-      }
-      tile[i * LDT + ll] = v;
-    }
-  } else {
-    for (int f = tid; f < TL * M; f += 256) {
-      const int i = f / TL, ll = f - i * TL;
-      T v = T(0);
-      if (ll < nl) {
-        const long l = l0 + ll;
-        long g;
-        if constexpr (AXIS == 1) g = (l / Z) * YZ + (long)i * Z + (l % Z);
-        else g = (long)i * YZ + l;
-        v = ch.in[g];
-        if (ch.pre) v *= ch.pre[g];
-      }
-      tile[i * LDT + ll] = v;
-    }
-  }
+#pragma unroll
+  for (int k = 0; k < KT; ++k) tile[lo[k]] = v[k];
   __syncthreads();
 
   // ---- out[j][t] = sum_i At[i][j] * tile[i][t] ----
@@ -109,8 +142,7 @@ __global__ __launch_bounds__(256) void axis_pass_kernel(PassParams<T> p) {
 #pragma unroll
     for (int c = 0; c < RT; ++c) acc[r][c] = T(0);
 
-#pragma unroll 4
-  for (int i = 0; i < M; ++i) {
+  auto body = [&](int i) {
     T a[RJ], b[RT];
 #pragma unroll
     for (int r = 0; r < RJ; ++r) {
@@ -124,6 +156,13 @@ __global__ __launch_bounds__(256) void axis_pass_kernel(PassParams<T> p) {
     for (int r = 0; r < RJ; ++r)
 #pragma unroll
       for (int c = 0; c < RT; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+  };
+  if (M == MP) {
+#pragma unroll 8
+    for (int i = 0; i < MP; ++i) body(i);
+  } else {
+#pragma unroll 2
+    for (int i = 0; i < M; ++i) body(i);
   }
   __syncthreads();
 #pragma unroll
@@ -133,30 +172,18 @@ __global__ __launch_bounds__(256) void axis_pass_kernel(PassParams<T> p) {
   __syncthreads();
 
   // ---- coalesced store, with the optional post multiply ----
-  if constexpr (AXIS == 2) {
-    const long base = l0 * Z;
-    for (int f = tid; f < TL * M; f += 256) {
-      const int ll = f / M, i = f - ll * M;
-      if (ll < nl) {
-        T v = tile[i * LDT + ll];
-        if (ch.post) v *= ch.post[base + f];
-        ch.out[base + f] = v;
-      }
-    }
-  } else {
-    for (int f = tid; f < TL * M; f += 256) {
-      const int i = f / TL, ll = f - i * TL;
-      if (ll < nl) {
-        const long l = l0 + ll;
-        long g;
-        if constexpr (AXIS == 1) g = (l / Z) * YZ + (long)i * Z + (l % Z);
-        else g = (long)i * YZ + l;
-        T v = tile[i * LDT + ll];
-        if (ch.post) v *= ch.post[g];
-        ch.out[g] = v;
-      }
-    }
+#pragma unroll
+  for (int k = 0; k < KT; ++k) v[k] = tile[lo[k]];
+  if (ch.post) {
+    T e[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.post[gi[k]] : T(0);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] *= e[k];
   }
+#pragma unroll
+  for (int k = 0; k < KT; ++k)
+    if (ok[k]) ch.out[gi[k]] = v[k];
 }
 
 // Padded length class for an axis of length M (0 if unsupported).

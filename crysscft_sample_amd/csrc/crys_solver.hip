@@ -238,6 +238,10 @@ int crys_solver_create(int kind, const size_t grid_log[3], const double translat
     for (int i = 0; i < 9; ++i) h->g[i] = translational_part[i];
   h->n_log = Nx * Ny * Nz;
   h->n_phy = h->nphy[0] * h->nphy[1] * h->nphy[2];
+  if (h->n_phy >= (size_t)1 << 31) {
+    delete h;
+    return fail(CRYS_ERR_UNSUPPORTED, "physical grid of %zu points exceeds 2^31", h->n_phy);
+  }
   for (int a = 0; a < 3; ++a)
     if (crys::pass_class((int)h->nphy[a]) == 0) {
       delete h;
