@@ -89,6 +89,8 @@ SIGNATURES = {
     "crys_axis_operator": (ctypes.c_int, [ctypes.c_int, _c_sz, ctypes.c_double, ctypes.c_double,
                                           _c_dp]),
     "crys_scft_time_step": (ctypes.c_int, [_c_vp, ctypes.c_int, _c_dp, _c_dp]),
+    "crys_scft_evaluate": (ctypes.c_int, [_c_vp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp,
+                                          _c_dp]),
 }
 
 _lib = None

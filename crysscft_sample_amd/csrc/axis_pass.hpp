@@ -186,6 +186,161 @@ __global__ __launch_bounds__(256) void axis_pass_kernel(PassParams<T> p) {
     if (ok[k]) ch.out[gi[k]] = v[k];
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core variant (MP <= 64): the same pass with the product on the
+// 16x16x4 MFMA of the element type (v_mfma_f64_16x16x4_f64 /
+// v_mfma_f32_16x16x4_f32).  Out(MP x TL) = D(MP x MP) * Tile(MP x TL): wave w
+// owns 16 output rows (j tile w % JT) and TPW column tiles of 16 lines; its
+// A operands (D[j][i] = At[i][j], 16 x 4 per step) stay in registers for the
+// whole pass, the B operands (4 x 16 tile slices) come from LDS.
+// Lane layout (16x16x4, one block): A lane l = A[l%16][l/16],
+// B lane l = B[l/16][l%16], D lane l register r = D[4(l/16)+r][l%16].
+// ---------------------------------------------------------------------------
+template <typename T>
+struct MfmaOp;
+template <>
+struct MfmaOp<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MfmaOp<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int MP>
+struct MfmaShape {
+  static constexpr int TL = MP <= 32 ? 64 : 32;  // lines per workgroup
+  static constexpr int LDT = TL + 16;            // LDS pitch: 4 B-rows hit distinct banks
+  static constexpr int JT = MP / 16;             // 16-row output tiles
+  static constexpr int TT = TL / 16;             // 16-line output tiles
+  static constexpr int KS = MP / 4;              // MFMA k steps
+  static constexpr int WPJ = JT >= 4 ? 1 : 4 / JT;  // waves sharing one row tile
+  static constexpr int TPW = TT / WPJ;           // line tiles per wave
+  static_assert(TT % WPJ == 0, "line tiles must split over the waves of a row tile");
+};
+
+template <typename T, int AXIS, int MP>
+__global__ __launch_bounds__(256) void axis_pass_mfma_kernel(PassParams<T> p) {
+  using S = MfmaShape<MP>;
+  using V4 = typename MfmaOp<T>::v4;
+  constexpr int NT = 256;
+  constexpr int TL = S::TL, LDT = S::LDT, JT = S::JT, KS = S::KS, WPJ = S::WPJ, TPW = S::TPW;
+  constexpr int KT = MP * TL / NT;
+  static_assert(MP * TL % NT == 0, "tile must split evenly over the workgroup");
+  __shared__ T tile[MP * LDT];
+
+  const PassChain<T> ch = p.c[blockIdx.y];
+  const int X = p.X, Y = p.Y, Z = p.Z;
+  const int M = AXIS == 0 ? X : (AXIS == 1 ? Y : Z);
+  const int YZ = Y * Z;
+  const int nlines = X * YZ / M;
+  const int l0 = blockIdx.x * TL;
+  const int nl = nlines - l0 < TL ? nlines - l0 : TL;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const bool yplane = AXIS == 1 && Z % TL == 0;
+  const int x0 = yplane ? l0 / Z : 0;
+  const int z0 = yplane ? l0 - x0 * Z : 0;
+
+  int gi[KT], lo[KT];
+  bool ok[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int f = tid + NT * k;
+    int i, ll;
+    if constexpr (AXIS == 2) {
+      ll = f / MP;
+      i = f - ll * MP;
+    } else {
+      i = f / TL;
+      ll = f - i * TL;
+    }
+    ok[k] = i < M && ll < nl;
+    lo[k] = i * LDT + ll;
+    int g;
+    if constexpr (AXIS == 2) {
+      g = (l0 + ll) * Z + i;
+    } else if constexpr (AXIS == 1) {
+      if (yplane) {
+        g = x0 * YZ + i * Z + z0 + ll;
+      } else {
+        const int l = l0 + ll, x = l / Z;
+        g = x * YZ + i * Z + (l - x * Z);
+      }
+    } else {
+      g = i * YZ + l0 + ll;
+    }
+    gi[k] = ok[k] ? g : 0;
+  }
+
+  // operands of this wave: rows jt*16.. of D, all k steps
+  const int jt = w % JT, sub = w / JT;
+  const bool active = w < JT * WPJ;
+  const int jrow = jt * 16 + (lane & 15);
+  T a[KS];
+#pragma unroll
+  for (int kk = 0; kk < KS; ++kk) {
+    const int i = 4 * kk + (lane >> 4);
+    a[kk] = (active && i < M && jrow < M) ? p.mat[i * M + jrow] : T(0);
+  }
+  T v[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) v[k] = ok[k] ? ch.in[gi[k]] : T(0);
+  if (ch.pre) {
+    T e[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.pre[gi[k]] : T(0);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] *= e[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KT; ++k) tile[lo[k]] = v[k];
+  __syncthreads();
+
+  V4 acc[TPW];
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    acc[u] = V4{T(0), T(0), T(0), T(0)};
+    if (active) {
+      const int col = (sub * TPW + u) * 16 + (lane & 15);
+#pragma unroll
+      for (int kk = 0; kk < KS; ++kk) {
+        const T b = tile[(4 * kk + (lane >> 4)) * LDT + col];
+        acc[u] = MfmaOp<T>::mma(a[kk], b, acc[u]);
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int col = (sub * TPW + u) * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tile[(jt * 16 + 4 * (lane >> 4) + r) * LDT + col] = acc[u][r];
+    }
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int k = 0; k < KT; ++k) v[k] = tile[lo[k]];
+  if (ch.post) {
+    T e[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.post[gi[k]] : T(0);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] *= e[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KT; ++k)
+    if (ok[k]) ch.out[gi[k]] = v[k];
+}
+
 // Padded length class for an axis of length M (0 if unsupported).
 inline int pass_class(int M) {
   if (M <= 0) return 0;
@@ -197,11 +352,24 @@ inline int pass_class(int M) {
   return 0;
 }
 
+// Which product the passes use: matrix cores (default) or the FMA kernel.
+enum class PassImpl { Mfma, Valu };
+PassImpl pass_impl();
+
 template <typename T, int AXIS, int MP>
 hipError_t launch_axis_pass_mp(const PassParams<T>& p, int nchains, hipStream_t s) {
-  using S = PassShape<MP>;
   const int M = AXIS == 0 ? p.X : (AXIS == 1 ? p.Y : p.Z);
   const long nlines = (long)p.X * p.Y * p.Z / M;
+  if constexpr (MP <= 64) {
+    if (pass_impl() == PassImpl::Mfma) {
+      using S = MfmaShape<MP>;
+      const long nblk = (nlines + S::TL - 1) / S::TL;
+      dim3 grid((unsigned)nblk, (unsigned)nchains, 1);
+      hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP>), grid, dim3(256), 0, s, p);
+      return hipGetLastError();
+    }
+  }
+  using S = PassShape<MP>;
   const long nblk = (nlines + S::TL - 1) / S::TL;
   dim3 grid((unsigned)nblk, (unsigned)nchains, 1);
   hipLaunchKernelGGL((axis_pass_kernel<T, AXIS, MP>), grid, dim3(256), 0, s, p);

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -57,10 +58,23 @@ unsigned grid_for(size_t n) {
 
 }  // namespace
 
+namespace crys {
+// CRYS_AXIS_PASS=valu selects the FMA pass kernel (for A/B timing); the
+// matrix-core kernel is the default.
+PassImpl pass_impl() {
+  static const PassImpl impl = [] {
+    const char* e = getenv("CRYS_AXIS_PASS");
+    return (e && strcmp(e, "valu") == 0) ? PassImpl::Valu : PassImpl::Mfma;
+  }();
+  return impl;
+}
+}  // namespace crys
+
 // One cached operator set: the three per-axis matrices for one ds value.
 struct OpSet {
   double ds;
-  void* dmat[3];  // device, element type of the solver's precision
+  void* dmat[3];     // device, element type of the solver's precision
+  void* dmat_dL[3];  // dD/dL per axis, built on first use (cell stress)
 };
 
 struct crys_solver {
@@ -85,6 +99,7 @@ struct crys_scft {
   double *wA = nullptr, *wB = nullptr, *ksi = nullptr, *phiA = nullptr, *phiB = nullptr;
   void *EA = nullptr, *EB = nullptr, *qf = nullptr, *qb = nullptr, *scr[2] = {nullptr, nullptr};
   double *qpart = nullptr, *epart = nullptr, *scal = nullptr, *trace = nullptr;
+  double *gA = nullptr, *gB = nullptr, *spart = nullptr, *sums = nullptr;  // gradient, lazily
   int* counter = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -98,31 +113,37 @@ size_t elem_size(int precision) { return precision == CRYS_FP32 ? sizeof(float) 
 
 void free_ops(crys_solver* h) {
   for (auto& o : h->ops)
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
       if (o.dmat[a]) (void)hipFree(o.dmat[a]);
+      if (o.dmat_dL[a]) (void)hipFree(o.dmat_dL[a]);
+    }
   h->ops.clear();
   h->cur = -1;
 }
 
-int build_ops(crys_solver* h, double ds, OpSet* out) {
-  out->ds = ds;
-  for (int a = 0; a < 3; ++a) out->dmat[a] = nullptr;
+int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3]) {
   for (int a = 0; a < 3; ++a) {
     std::vector<double> At;
     const int M = (int)h->nphy[a];
-    if (h->kind == CRYS_PMMM_DCT) At = crys::pmmm_axis_operator(M, h->cell[a], ds);
-    else if (h->kind == CRYS_P1) At = crys::p1_axis_operator((int)h->nlog[a], h->cell[a], ds);
-    else At = crys::m3_axis_operator((int)h->nlog[a], h->cell[a], ds);
+    if (h->kind == CRYS_PMMM_DCT) At = crys::pmmm_axis_operator(M, h->cell[a], ds, dL);
+    else if (h->kind == CRYS_P1) At = crys::p1_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
+    else At = crys::m3_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
     const size_t cnt = (size_t)M * M;
-    HIPCHK(hipMalloc(&out->dmat[a], cnt * elem_size(h->precision)));
+    HIPCHK(hipMalloc(&dst[a], cnt * elem_size(h->precision)));
     if (h->precision == CRYS_FP32) {
       std::vector<float> f(At.begin(), At.end());
-      HIPCHK(hipMemcpy(out->dmat[a], f.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(dst[a], f.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
     } else {
-      HIPCHK(hipMemcpy(out->dmat[a], At.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(dst[a], At.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
     }
   }
   return CRYS_OK;
+}
+
+int build_ops(crys_solver* h, double ds, OpSet* out) {
+  out->ds = ds;
+  for (int a = 0; a < 3; ++a) out->dmat[a] = out->dmat_dL[a] = nullptr;
+  return upload_axis_ops(h, ds, false, out->dmat);
 }
 
 // One contour step E D E (or a bare D with E == nullptr) for 1 or 2 chains:
@@ -458,9 +479,10 @@ void scft_free(crys_scft* h) {
   if (h->graph) (void)hipGraphDestroy(h->graph);
   h->exec = nullptr;
   h->graph = nullptr;
-  void* bufs[] = {h->wA,    h->wB,    h->ksi,    h->phiA,   h->phiB,  h->EA,
-                  h->EB,    h->qf,    h->qb,     h->scr[0], h->scr[1], h->qpart,
-                  h->epart, h->scal,  h->trace,  h->counter};
+  void* bufs[] = {h->wA,    h->wB,   h->ksi,   h->phiA,    h->phiB,   h->EA,
+                  h->EB,    h->qf,   h->qb,    h->scr[0],  h->scr[1], h->qpart,
+                  h->epart, h->scal, h->trace, h->counter, h->gA,     h->gB,
+                  h->spart, h->sums};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
 }
@@ -470,7 +492,7 @@ void scft_free(crys_scft* h) {
 // chain q_B^dagger then q_A^dagger (joined at slot NsA, :490); both advance
 // in the same launches.
 template <typename T>
-int scft_record_iteration(crys_scft* h, hipStream_t s) {
+int scft_record_iteration(crys_scft* h, hipStream_t s, double acc_mix) {
   const crys_solver* sv = h->sv;
   const OpSet& op = sv->ops[sv->cur];
   const size_t n = h->n, Ns = h->Ns, NsA = h->NsA;
@@ -496,12 +518,47 @@ int scft_record_iteration(crys_scft* h, hipStream_t s) {
   hipLaunchKernelGGL((crys::phi_energy_mix_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
                      (const T*)qf, (const T*)qb, (const double*)h->scal, h->phiA, h->phiB, h->wA,
                      h->wB, h->ksi, (T*)h->EA, (T*)h->EB, h->epart, n, h->NsA, h->NsB, h->xN,
-                     h->acc, h->ds);
+                     acc_mix, h->ds);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(crys::energy_finalize_kernel, dim3(1), dim3(crys::kRedThreads), 0, s,
                      (const double*)h->epart, nb, (const double*)h->scal, h->trace, h->counter,
                      kTraceCap, n);
   HIPCHK(hipGetLastError());
+  return CRYS_OK;
+}
+
+// d Q / d L_a summed over the contour: for every step s and axis a,
+// < E_s q^dagger_{s+1}, (dD_a (x) D_b (x) D_c) E_s q_s > with the derivative
+// operator on axis a.  Partials go to h->spart[(3 s + a) * nb + block].
+template <typename T>
+int scft_record_stress(crys_scft* h, hipStream_t s) {
+  const crys_solver* sv = h->sv;
+  const OpSet& op = sv->ops[sv->cur];
+  const size_t n = h->n, Ns = h->Ns, NsA = h->NsA;
+  const T* qf = (const T*)h->qf;
+  const T* qb = (const T*)h->qb;
+  T* w = (T*)h->scr[0];
+  const int nb = crys::red_blocks(n);
+  crys::PassParams<T> p{};
+  p.X = (int)sv->nphy[0];
+  p.Y = (int)sv->nphy[1];
+  p.Z = (int)sv->nphy[2];
+  for (size_t t = 0; t < Ns; ++t) {
+    const T* E = (const T*)(t < NsA ? h->EA : h->EB);
+    for (int a = 0; a < 3; ++a) {
+      // z pass (pre E), then y, then x; the axis a pass uses dD/dL_a
+      const int order[3] = {2, 1, 0};
+      for (int q = 0; q < 3; ++q) {
+        const int ax = order[q];
+        p.c[0] = {q == 0 ? qf + t * n : w, w, q == 0 ? E : nullptr, nullptr};
+        p.mat = (const T*)(ax == a ? op.dmat_dL[ax] : op.dmat[ax]);
+        HIPCHK(crys::launch_axis_pass<T>(ax, p, 1, s));
+      }
+      hipLaunchKernelGGL((crys::dot3_partial_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
+                         (const T*)w, E, qb + (t + 1) * n, h->spart + (3 * t + a) * nb, n);
+      HIPCHK(hipGetLastError());
+    }
+  }
   return CRYS_OK;
 }
 
@@ -512,8 +569,8 @@ int scft_build_graph(crys_scft* h) {
   h->exec = nullptr;
   h->graph = nullptr;
   HIPCHK(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
-  int r = sv->precision == CRYS_FP32 ? scft_record_iteration<float>(h, sv->stream)
-                                     : scft_record_iteration<double>(h, sv->stream);
+  int r = sv->precision == CRYS_FP32 ? scft_record_iteration<float>(h, sv->stream, h->acc)
+                                     : scft_record_iteration<double>(h, sv->stream, h->acc);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(sv->stream, &g);
   if (r != CRYS_OK) {
@@ -711,6 +768,67 @@ int crys_diblock_scft_simple_iterate(crys_solver_t* solver, double fA, double ds
   crys_scft_destroy(h);
   g_err = keep;
   return r;
+}
+
+int crys_scft_evaluate(crys_scft_t* h, const double* wA, const double* wB, double* H,
+                       double* dH_dwA, double* dH_dwB, double dH_dL[3], double* trace_row) {
+  if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
+  if ((wA == nullptr) != (wB == nullptr))
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "pass both fields or neither");
+  if (wA) CHK(crys_scft_set_fields(h, wA, wB));
+  if (!h->fields_set) return fail(CRYS_ERR_INVALID_ARGUMENT, "no fields set");
+  crys_solver* sv = h->sv;
+  if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
+  OpSet& op = sv->ops[sv->cur];
+  hipStream_t s = sv->stream;
+  const size_t n = h->n, Ns = h->Ns;
+  const int nb = crys::red_blocks(n);
+  const bool stress = dH_dL != nullptr;
+  if (!h->gA) HIPCHK(hipMalloc((void**)&h->gA, n * 8));
+  if (!h->gB) HIPCHK(hipMalloc((void**)&h->gB, n * 8));
+  if (!h->sums) HIPCHK(hipMalloc((void**)&h->sums, (1 + 3 * Ns) * 8));
+  if (stress) {
+    if (!h->spart) HIPCHK(hipMalloc((void**)&h->spart, 3 * Ns * nb * 8));
+    if (!op.dmat_dL[0]) CHK(upload_axis_ops(sv, op.ds, true, op.dmat_dL));
+  }
+  hipLaunchKernelGGL(crys::set_int_kernel, dim3(1), dim3(1), 0, s, h->counter, 0);
+  HIPCHK(hipGetLastError());
+  const bool f32 = sv->precision == CRYS_FP32;
+  CHK(f32 ? scft_record_iteration<float>(h, s, 0.0) : scft_record_iteration<double>(h, s, 0.0));
+  const double cA = h->ds * (double)h->NsA / h->fA, cB = h->ds * (double)h->NsB / h->fB;
+  hipLaunchKernelGGL(crys::hgrad_kernel, dim3(nb), dim3(crys::kRedThreads), 0, s,
+                     (const double*)h->wA, (const double*)h->wB, (const double*)h->phiA,
+                     (const double*)h->phiB, h->gA, h->gB, h->qpart, n, h->xN, cA, cB);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(crys::sum_partials_kernel, dim3(1), dim3(crys::kRedThreads), 0, s,
+                     (const double*)h->qpart, nb, h->sums);
+  HIPCHK(hipGetLastError());
+  if (stress) {
+    CHK(f32 ? scft_record_stress<float>(h, s) : scft_record_stress<double>(h, s));
+    hipLaunchKernelGGL(crys::sum_partials_kernel, dim3((unsigned)(3 * Ns)), dim3(crys::kRedThreads),
+                       0, s, (const double*)h->spart, nb, h->sums + 1);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<double> sums(1 + 3 * Ns);
+  double row[CRYS_TRACE_COLS];
+  HIPCHK(hipMemcpyAsync(row, h->trace, sizeof(row), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(sums.data(), h->sums, (stress ? 1 + 3 * Ns : 1) * 8,
+                        hipMemcpyDeviceToHost, s));
+  if (dH_dwA) HIPCHK(hipMemcpyAsync(dH_dwA, h->gA, n * 8, hipMemcpyDeviceToHost, s));
+  if (dH_dwB) HIPCHK(hipMemcpyAsync(dH_dwB, h->gB, n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const double Q = row[CRYS_TRACE_Q];
+  if (trace_row) memcpy(trace_row, row, sizeof(row));
+  if (H) *H = sums[0] / (double)n - log(Q);
+  if (stress) {
+    for (int a = 0; a < 3; ++a) {
+      double acc = 0.0;
+      for (size_t t = 0; t < Ns; ++t) acc += sums[1 + 3 * t + a];
+      dH_dL[a] = -acc / ((double)n * Q);
+    }
+  }
+  if (!std::isfinite(Q) || !(Q > 0)) return fail(CRYS_ERR_DIVERGED, "non-finite or zero Q");
+  return CRYS_OK;
 }
 
 int crys_axis_operator(int kind, size_t n_logical, double L, double ds, double* out) {

@@ -195,6 +195,54 @@ __global__ __launch_bounds__(kRedThreads) void energy_finalize_kernel(const doub
   }
 }
 
+// Partial sums of u * E * v (the stress inner products).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void dot3_partial_kernel(const T* u, const T* E,
+                                                                   const T* v, double* part,
+                                                                   size_t n) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kRedThreads)
+    acc += (double)u[i] * ((double)E[i] * (double)v[i]);
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Gradient of the field-theoretic free energy
+//   H[wA, wB] = (1/n) sum_r [ (wA-wB)^2 / (4 xN) - (wA+wB)/2 + xN/4 ] - ln Q
+// from the densities of the same fields: d(-ln Q)/dwA(r) = cA phiA(r) / n with
+// cA = ds NsA / fA (and B alike).  Partials of the field part go to part.
+__global__ __launch_bounds__(kRedThreads) void hgrad_kernel(const double* wA, const double* wB,
+                                                            const double* phiA,
+                                                            const double* phiB, double* gA,
+                                                            double* gB, double* part, size_t n,
+                                                            double xN, double cA, double cB) {
+  __shared__ double sh[kRedThreads];
+  const double inv_n = 1.0 / (double)n;
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kRedThreads) {
+    const double a = wA[i], b = wB[i], d = a - b;
+    acc += d * d / (4 * xN) - (a + b) / 2 + xN / 4;
+    gA[i] = (d / (2 * xN) - 0.5 + cA * phiA[i]) * inv_n;
+    gB[i] = (-d / (2 * xN) - 0.5 + cB * phiB[i]) * inv_n;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[k] = sum of part[k * nparts + b] over b, one workgroup per k.
+__global__ __launch_bounds__(kRedThreads) void sum_partials_kernel(const double* part, int nparts,
+                                                                   double* out) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  const double* p = part + (size_t)blockIdx.x * nparts;
+  for (int i = threadIdx.x; i < nparts; i += kRedThreads) acc += p[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 }  // namespace crys

@@ -19,15 +19,16 @@ std::vector<long double> cos_table(long d) {
   return t;
 }
 
-// exp(-ds (2 pi k / L)^2), the per-axis factor of K.
-long double kfactor(long k, double L, double ds) {
+// exp(-ds (2 pi k / L)^2), the per-axis factor of K, or its L derivative.
+long double kfactor(long k, double L, double ds, bool dL) {
   long double kk = 2.0L * kPiL * (long double)k / (long double)L;
-  return expl(-(long double)ds * kk * kk);
+  long double K = expl(-(long double)ds * kk * kk);
+  return dL ? K * 2.0L * (long double)ds * kk * kk / (long double)L : K;
 }
 
-std::vector<long double> p1_dense(int N, double L, double ds) {
+std::vector<long double> p1_dense(int N, double L, double ds, bool dL) {
   std::vector<long double> K(N);
-  for (int k = 0; k < N; ++k) K[k] = kfactor(k < (N + 1) / 2 ? k : N - k, L, ds);
+  for (int k = 0; k < N; ++k) K[k] = kfactor(k < (N + 1) / 2 ? k : N - k, L, ds, dL);
   std::vector<long double> ct = cos_table(N);
   // circulant: D[j][j'] = c(j - j') with c(d) = (1/N) sum_k K(k) cos(2 pi k d / N)
   std::vector<long double> c(N);
@@ -44,11 +45,11 @@ std::vector<long double> p1_dense(int N, double L, double ds) {
 
 }  // namespace
 
-std::vector<double> pmmm_axis_operator(int M, double L, double ds) {
+std::vector<double> pmmm_axis_operator(int M, double L, double ds, bool dL) {
   // cos(pi k (2j+1) / (2M)) = cos(2 pi m / (4M)), m = k (2j+1) mod 4M
   std::vector<long double> ct = cos_table(4L * M);
   std::vector<long double> K(M);
-  for (int k = 0; k < M; ++k) K[k] = kfactor(k, L, ds);
+  for (int k = 0; k < M; ++k) K[k] = kfactor(k, L, ds, dL);
   std::vector<double> At((size_t)M * M);
   for (int j = 0; j < M; ++j)
     for (int jp = 0; jp < M; ++jp) {
@@ -65,17 +66,17 @@ std::vector<double> pmmm_axis_operator(int M, double L, double ds) {
   return At;
 }
 
-std::vector<double> p1_axis_operator(int N, double L, double ds) {
-  std::vector<long double> D = p1_dense(N, L, ds);
+std::vector<double> p1_axis_operator(int N, double L, double ds, bool dL) {
+  std::vector<long double> D = p1_dense(N, L, ds, dL);
   std::vector<double> At((size_t)N * N);
   for (int j = 0; j < N; ++j)
     for (int i = 0; i < N; ++i) At[(size_t)i * N + j] = (double)D[(size_t)j * N + i];
   return At;
 }
 
-std::vector<double> m3_axis_operator(int N, double L, double ds) {
+std::vector<double> m3_axis_operator(int N, double L, double ds, bool dL) {
   const int M = N / 2;
-  std::vector<long double> D = p1_dense(N, L, ds);
+  std::vector<long double> D = p1_dense(N, L, ds, dL);
   std::vector<double> At((size_t)M * M);
   for (int j = 0; j < M; ++j)
     for (int i = 0; i < M; ++i) {

@@ -25,9 +25,11 @@
 namespace crys {
 
 // Each builder returns At with At[i * M + j] = D[j][i], i.e. out_j =
-// sum_i At[i*M + j] * in_i (the layout the device pass streams).
-std::vector<double> pmmm_axis_operator(int M, double L, double ds);
-std::vector<double> p1_axis_operator(int N, double L, double ds);
-std::vector<double> m3_axis_operator(int N_logical, double L, double ds);
+// sum_i At[i*M + j] * in_i (the layout the device pass streams).  With
+// dL = true it returns dD/dL instead: K_k is replaced by
+// dK_k/dL = K_k * 2 ds (2 pi k)^2 / L^3 (the cell stress of the operator).
+std::vector<double> pmmm_axis_operator(int M, double L, double ds, bool dL = false);
+std::vector<double> p1_axis_operator(int N, double L, double ds, bool dL = false);
+std::vector<double> m3_axis_operator(int N_logical, double L, double ds, bool dL = false);
 
 }  // namespace crys

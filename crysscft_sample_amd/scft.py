@@ -50,6 +50,7 @@ class SCFT:
                                      float(acceptance), ctypes.byref(h)))
         self._h = h
         self.solver = solver
+        self.fA, self.ds, self.xN = float(fA), float(ds), float(xN)
         self.shape = solver.get_grid_size_phy()
         a, b = ctypes.c_size_t(), ctypes.c_size_t()
         check(lib().crys_scft_get_contour(h, ctypes.byref(a), ctypes.byref(b)))
@@ -85,6 +86,26 @@ class SCFT:
         phiA, phiB, wA, wB = (np.empty(n) for _ in range(4))
         check(lib().crys_scft_get_fields(self._h, dptr(phiA), dptr(phiB), dptr(wA), dptr(wB)))
         return phiA, phiB, wA, wB
+
+    def evaluate(self, wA=None, wB=None, stress=True):
+        """H[wA, wB], dH/dwA, dH/dwB and the cell stress dH/dL (crys_scft_evaluate).
+
+        Returns a dict with ``H``, ``dH_dwA``, ``dH_dwB`` (physical shape),
+        ``dH_dL`` (3,) or None, and ``trace_row`` (the print_energy terms)."""
+        shape = self.shape
+        a = b = None
+        if wA is not None:
+            a = np.ascontiguousarray(wA, dtype=np.float64).reshape(-1)
+            b = np.ascontiguousarray(wB, dtype=np.float64).reshape(-1)
+        H = ctypes.c_double()
+        gA = np.empty(shape)
+        gB = np.empty(shape)
+        dL = np.empty(3) if stress else None
+        row = np.empty(_lib.TRACE_COLS)
+        check(lib().crys_scft_evaluate(self._h, None if a is None else dptr(a),
+                                       None if b is None else dptr(b), ctypes.byref(H), dptr(gA),
+                                       dptr(gB), None if dL is None else dptr(dL), dptr(row)))
+        return {"H": H.value, "dH_dwA": gA, "dH_dwB": gB, "dH_dL": dL, "trace_row": row}
 
     def time_step(self, reps: int = 50):
         """(ms per contour step of both chains, ms per axis-pass launch)."""

@@ -169,6 +169,29 @@ CRYS_API int crys_diblock_scft_simple_iterate(crys_solver_t* solver, double fA, 
                                      double* trace, double* phiA, double* phiB, double* wA_out,
                                      double* wB_out);
 
+/* ---- free energy, gradient and cell stress ------------------------------ */
+
+/* Evaluates, for the fields (wA, wB) (host arrays of space_size_phy doubles;
+ * both NULL = the handle's current fields) on the solver's current cell:
+ *   H = (1/n) sum_r [ (wA-wB)^2 / (4 xN) - (wA+wB)/2 + xN/4 ] - ln Q[wA, wB],
+ * the field-theoretic free energy of the incompressible melt (it equals the
+ * FreeEnergy of print_energy, crysSCFT_sample.cpp:398-424, at a saddle point,
+ * where ksi = (wA+wB-xN)/2 as calc_ksi :341-346 sets it), and
+ *   dH_dwA[r] = ((wA-wB)/(2 xN) - 1/2 + (ds NsA / fA) phiA[r]) / n, B alike,
+ * from the forward and backward (adjoint) propagators of those fields, and
+ *   dH_dL[a] = -(1/Q) dQ/dL_a = -(1/(n Q)) sum_s < E q^+_{s+1}, dD/dL_a E q_s >,
+ * the cell stress used for box relaxation, from the adjoint inner products
+ * with the L derivative of the per-axis operator.  For a PmmmDCT solver
+ * without CRYS_FLAG_PMMM_ORTHORHOMBIC every axis uses cell_para[0]
+ * (solver_Pmmm.h:101) and the derivative with respect to that single length
+ * is dH_dL[0] + dH_dL[1] + dH_dL[2].  trace_row (7 doubles, enum
+ * crys_trace_col) receives the print_energy terms of the same fields.  The
+ * fields are not mixed.  Any output may be NULL; the stress is computed only
+ * when dH_dL is not NULL.  Synchronous. */
+CRYS_API int crys_scft_evaluate(crys_scft_t* h, const double* wA, const double* wB, double* H,
+                                double* dH_dwA, double* dH_dwB, double dH_dL[3],
+                                double* trace_row);
+
 /* ---- introspection (host only, no device needed) ------------------------ */
 
 /* The per-axis operator the device applies for one axis of a solver: D of

@@ -1,0 +1,193 @@
+"""Free energy H[w; L], its field gradient and the cell stress (needs an MI355X).
+
+Checks, for crys_scft_evaluate:
+  - dH/dw against the densities of the oracle (d lnQ / dwA = -(ds NsA / fA)
+    phiA / n, the reference's own phi fields) to 1e-10;
+  - dH/dw and dH/dL against central finite differences of the GPU's own H
+    (h = 1e-5) to 1e-6 relative;
+  - dH/dL against central finite differences of the oracle's H, and across
+    the three solvers on mirror-symmetric fields;
+  - the torch.autograd binding against the same numbers.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel_err
+import crysscft_sample_amd as C
+from crysscft_sample_amd import _lib
+from crysscft_sample_amd import fields as F
+
+pytestmark = pytest.mark.gpu
+
+# Differentiation workload, reduced: orthorhombic cell, Ns = 128 (35 + 92).
+N = (32, 32, 32)
+CELL = (4.0, 4.1, 4.3)
+FA, XN, DS = 0.28, 22.0, 1.0 / 128
+
+
+def say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def oracle_H(oracle, name, wA, wB, cell):
+    """H and the densities from the oracle's P1 solver on the full mirror-
+    symmetric fields (the oracle's Pmmm keeps the reference's cubic cell)."""
+    s = oracle.Solver("P1", N)
+    fa, fb = F.expand(wA, name), F.expand(wB, name)
+    tr, _, _, phiA, phiB = s.scft(FA, DS, XN, cell, 0.0, 1, fa, fb)
+    d = fa - fb
+    field = np.mean(d * d / (4 * XN) - (fa + fb) / 2 + XN / 4)
+    return field - np.log(tr[0, 6]), F.extract(phiA, name), F.extract(phiB, name)
+
+
+def make_run(name, cell=CELL):
+    if name == "PmmmDCT":
+        s = C.SolverPmmmDCT(N, orthorhombic=True)
+    else:
+        s = C.SOLVERS[name](N)
+    run = C.SCFT(s, FA, DS, XN, cell, 0.0)
+    return s, run
+
+
+def fields(name):
+    wA, wB = F.bcc_fields(N, CELL, FA, XN)
+    return F.extract(wA, name), F.extract(wB, name)
+
+
+@pytest.mark.parametrize("name", ["PmmmDCT", "3m", "P1"])
+def test_gradient_equals_oracle_densities(oracle, name):
+    say("gradient vs oracle densities", name)
+    wA, wB = fields(name)
+    s, run = make_run(name)
+    res = run.evaluate(wA, wB, stress=False)
+    H, phiA, phiB = oracle_H(oracle, name, wA, wB, CELL)
+    assert run.NsA == 35 and run.NsB == 92
+    n = wA.size
+    cA, cB = DS * 35 / FA, DS * 92 / (1 - FA)
+    gA = ((wA - wB) / (2 * XN) - 0.5 + cA * phiA) / n
+    gB = (-(wA - wB) / (2 * XN) - 0.5 + cB * phiB) / n
+    assert rel_err(res["dH_dwA"], gA) < 1e-10
+    assert rel_err(res["dH_dwB"], gB) < 1e-10
+    assert abs(res["H"] / H - 1) < 1e-11
+
+
+def test_field_gradient_central_differences():
+    name = "PmmmDCT"
+    wA, wB = fields(name)
+    s, run = make_run(name)
+    g = run.evaluate(wA, wB, stress=False)
+    h = 1e-5
+    for idx in [(0, 0, 0), (3, 9, 14), (15, 1, 7)]:
+        for key, base in (("dH_dwA", 0), ("dH_dwB", 1)):
+            hp = []
+            for sgn in (1, -1):
+                a, b = wA.copy(), wB.copy()
+                (a if base == 0 else b)[idx] += sgn * h
+                hp.append(run.evaluate(a, b, stress=False)["H"])
+            fd = (hp[0] - hp[1]) / (2 * h)
+            an = g[key][idx]
+            say(f"  {key}{idx}: fd {fd:.10e} adjoint {an:.10e}")
+            assert abs(fd / an - 1) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["PmmmDCT", "P1"])
+def test_stress_central_differences(name):
+    wA, wB = fields(name)
+    s, run = make_run(name)
+    g = run.evaluate(wA, wB, stress=True)
+    h = 1e-5
+    for a in range(3):
+        hp = []
+        for sgn in (1, -1):
+            cell = list(CELL)
+            cell[a] += sgn * h
+            s.set_cell_para(cell)
+            s.set_contour_step(DS)
+            hp.append(run.evaluate(wA, wB, stress=False)["H"])
+        s.set_cell_para(CELL)
+        s.set_contour_step(DS)
+        fd = (hp[0] - hp[1]) / (2 * h)
+        say(f"  {name} dH/dL{a}: fd {fd:.10e} adjoint {g['dH_dL'][a]:.10e}")
+        assert abs(fd / g["dH_dL"][a] - 1) < 1e-6
+
+
+def test_stress_matches_oracle_and_other_solvers(oracle):
+    wAf, wBf = F.bcc_fields(N, CELL, FA, XN)
+    stress = {}
+    for name in ("PmmmDCT", "3m", "P1"):
+        s, run = make_run(name)
+        stress[name] = run.evaluate(F.extract(wAf, name), F.extract(wBf, name))["dH_dL"]
+    for name in ("3m", "P1"):
+        assert rel_err(stress[name], stress["PmmmDCT"]) < 1e-10
+    wA, wB = F.extract(wAf, "P1"), F.extract(wBf, "P1")
+    h = 1e-5
+    for a in range(3):
+        hp = []
+        for sgn in (1, -1):
+            cell = list(CELL)
+            cell[a] += sgn * h
+            hp.append(oracle_H(oracle, "P1", wA, wB, cell)[0])
+        fd = (hp[0] - hp[1]) / (2 * h)
+        assert abs(fd / stress["P1"][a] - 1) < 1e-6
+
+
+def test_evaluate_at_uniform_fields_has_zero_stress():
+    s, run = make_run("P1")
+    shape = s.get_grid_size_phy()
+    res = run.evaluate(np.full(shape, 0.7), np.full(shape, -0.4))
+    assert np.abs(res["dH_dL"]).max() < 1e-12
+    Q = np.exp(-DS * (0.7 * 35 - 0.4 * 92))
+    assert abs(res["trace_row"][6] / Q - 1) < 1e-13
+
+
+def test_fp32_gradient_error_is_bounded():
+    name = "PmmmDCT"
+    wA, wB = fields(name)
+    _, r64 = make_run(name)
+    s32 = C.SolverPmmmDCT(N, precision="fp32", orthorhombic=True)
+    r32 = C.SCFT(s32, FA, DS, XN, CELL, 0.0)
+    a, b = r64.evaluate(wA, wB), r32.evaluate(wA, wB)
+    say(f"  fp32: |dH/H| {abs(b['H'] / a['H'] - 1):.2e}, grad {rel_err(b['dH_dwA'], a['dH_dwA']):.2e},"
+        f" stress {rel_err(b['dH_dL'], a['dH_dL']):.2e}")
+    assert abs(b["H"] / a["H"] - 1) < 1e-5
+    assert rel_err(b["dH_dwA"], a["dH_dwA"]) < 1e-4
+    assert rel_err(b["dH_dL"], a["dH_dL"]) < 1e-3
+
+
+TORCH_SCRIPT = r"""
+import sys, numpy as np
+import torch                                  # torch's HIP runtime first
+sys.path.insert(0, sys.argv[1])
+import crysscft_sample_amd as C
+from crysscft_sample_amd import fields as F
+from crysscft_sample_amd.autograd import free_energy
+N, cell, fA, xN, ds = (32, 32, 32), (4.0, 4.1, 4.3), 0.28, 22.0, 1 / 128
+s = C.SolverPmmmDCT(N, orthorhombic=True)
+run = C.SCFT(s, fA, ds, xN, cell, 0.0)
+wA, wB = F.bcc_fields(N, cell, fA, xN)
+wA, wB = F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT")
+ref = run.evaluate(wA, wB)
+tA = torch.tensor(wA, dtype=torch.float64, requires_grad=True)
+tB = torch.tensor(wB, dtype=torch.float64, requires_grad=True)
+tL = torch.tensor(cell, dtype=torch.float64, requires_grad=True)
+H = free_energy(tA, tB, tL, run)
+H.backward()
+ok = (abs(H.item() - ref["H"]) <= 1e-14 * abs(ref["H"])
+      and np.array_equal(tA.grad.numpy(), ref["dH_dwA"])
+      and np.array_equal(tB.grad.numpy(), ref["dH_dwB"])
+      and np.array_equal(tL.grad.numpy(), ref["dH_dL"]))
+print("torch autograd ok" if ok else "torch autograd MISMATCH", flush=True)
+sys.exit(0 if ok else 1)
+"""
+
+
+def test_torch_autograd_function():
+    say("torch.autograd binding (subprocess, torch imported first)")
+    r = subprocess.run([sys.executable, "-c", TORCH_SCRIPT, ROOT], capture_output=True, text=True,
+                       timeout=240)
+    say(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
