@@ -16,7 +16,7 @@ P1, M3, PMMM_DCT = 0, 1, 2
 FP64, FP32 = 0, 1
 FLAG_PMMM_ORTHORHOMBIC = 1
 OK, ERR_INVALID_ARGUMENT, ERR_BAD_GRID, ERR_NO_CONTOUR_STEP = 0, 1, 2, 3
-ERR_UNSUPPORTED, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_DIVERGED = 4, 5, 6, 7
+ERR_UNSUPPORTED, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_DIVERGED, ERR_IO = 4, 5, 6, 7, 8
 TRACE_COLS = 7
 TRACE_NAMES = ("F", "enthalpic", "minus_lnQ", "entropic", "ksi", "max_incomp", "Q")
 
@@ -28,6 +28,7 @@ STATUS_NAMES = {
     ERR_HIP: "CRYS_ERR_HIP",
     ERR_OUT_OF_MEMORY: "CRYS_ERR_OUT_OF_MEMORY",
     ERR_DIVERGED: "CRYS_ERR_DIVERGED",
+    ERR_IO: "CRYS_ERR_IO",
 }
 
 
@@ -89,6 +90,11 @@ SIGNATURES = {
     "crys_axis_operator": (ctypes.c_int, [ctypes.c_int, _c_sz, ctypes.c_double, ctypes.c_double,
                                           _c_dp]),
     "crys_scft_time_step": (ctypes.c_int, [_c_vp, ctypes.c_int, _c_dp, _c_dp]),
+    "crys_output_fields": (ctypes.c_int, [_c_dp, _c_dp, _c_dp, _c_dp, _c_sz, ctypes.c_char_p,
+                                          ctypes.c_int]),
+    "crys_input_fields": (ctypes.c_int, [_c_dp, _c_dp, _c_dp, _c_dp, _c_sz, ctypes.c_char_p]),
+    "crys_extract_fields": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_c_sz), _c_dp, _c_dp]),
+    "crys_expand_fields": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_c_sz), _c_dp, _c_dp]),
     "crys_scft_evaluate": (ctypes.c_int, [_c_vp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp,
                                           _c_dp]),
 }

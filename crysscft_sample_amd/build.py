@@ -18,7 +18,7 @@ LIB_NAME = "libcrys_solver.so"
 LIB_PATH = os.path.join(PKG, LIB_NAME)
 
 HIP_SOURCES = ["crys_solver.hip"]
-CXX_SOURCES = ["operators.cpp"]
+CXX_SOURCES = ["operators.cpp", "field_io.cpp"]
 
 ARCH = os.environ.get("CRYS_OFFLOAD_ARCH", "gfx950")
 

@@ -59,6 +59,8 @@ unsigned grid_for(size_t n) {
 }  // namespace
 
 namespace crys {
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
 // CRYS_AXIS_PASS=valu selects the FMA pass kernel (for A/B timing); the
 // matrix-core kernel is the default.
 PassImpl pass_impl() {

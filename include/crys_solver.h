@@ -56,7 +56,8 @@ enum crys_status {
   CRYS_ERR_UNSUPPORTED = 4,      /* a combination this build does not run */
   CRYS_ERR_HIP = 5,              /* a HIP runtime call failed */
   CRYS_ERR_OUT_OF_MEMORY = 6,
-  CRYS_ERR_DIVERGED = 7          /* non-finite free energy or Q */
+  CRYS_ERR_DIVERGED = 7,         /* non-finite free energy or Q */
+  CRYS_ERR_IO = 8                /* a field file could not be read or written */
 };
 
 /* Description of the last error on this thread ("" if none). */
@@ -168,6 +169,27 @@ CRYS_API int crys_diblock_scft_simple_iterate(crys_solver_t* solver, double fA, 
                                      size_t nstep, const double* wA_init, const double* wB_init,
                                      double* trace, double* phiA, double* phiB, double* wA_out,
                                      double* wB_out);
+
+/* ---- field files and grid maps (host only) ------------------------------ */
+
+/* output_fields (crysSCFT_sample.cpp:271-282): one line "phiA phiB wA wB" per
+ * point.  digits <= 0 writes the reference's default 6 significant digits;
+ * 17 makes the file an exact restart point. */
+CRYS_API int crys_output_fields(const double* phiA, const double* phiB, const double* wA,
+                                const double* wB, size_t size, const char* filename, int digits);
+/* input_fields (crysSCFT_sample.cpp:284-292).  A missing or short file is
+ * CRYS_ERR_IO (the reference leaves the arrays unchanged silently). */
+CRYS_API int crys_input_fields(double* phiA, double* phiB, double* wA, double* wB, size_t size,
+                               const char* filename);
+/* The reduced field a solver works on, from the full logical grid
+ * (crysSCFT_sample.cpp:202-258): P1 copies, 3m takes f[2x][2y][2z], PmmmDCT
+ * takes the first octant. */
+CRYS_API int crys_extract_fields(int kind, const size_t grid_log[3], const double* full,
+                                 double* reduced);
+/* Inverse of crys_extract_fields for fields with the mirrors f(N-1-j) = f(j):
+ * Pmmm reflects the octant, 3m fills odd sites from f[2j+1] = e[M-1-j]. */
+CRYS_API int crys_expand_fields(int kind, const size_t grid_log[3], const double* reduced,
+                                double* full);
 
 /* ---- free energy, gradient and cell stress ------------------------------ */
 
