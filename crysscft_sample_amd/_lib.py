@@ -95,6 +95,13 @@ SIGNATURES = {
     "crys_input_fields": (ctypes.c_int, [_c_dp, _c_dp, _c_dp, _c_dp, _c_sz, ctypes.c_char_p]),
     "crys_extract_fields": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_c_sz), _c_dp, _c_dp]),
     "crys_expand_fields": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_c_sz), _c_dp, _c_dp]),
+    "crys_batch_create": (ctypes.c_int, [_c_vp, _c_sz, ctypes.c_int, ctypes.POINTER(_c_vp)]),
+    "crys_batch_destroy": (ctypes.c_int, [_c_vp]),
+    "crys_batch_set_fields": (ctypes.c_int, [_c_vp, _c_sz, _c_dp, _c_dp]),
+    "crys_batch_run": (ctypes.c_int, [_c_vp, _c_sz, _c_dp]),
+    "crys_batch_get_fields": (ctypes.c_int, [_c_vp, _c_sz, _c_dp, _c_dp, _c_dp, _c_dp]),
+    "crys_batch_get_info": (ctypes.c_int, [_c_vp, _c_sz, ctypes.POINTER(_c_sz),
+                                           ctypes.POINTER(_c_sz), ctypes.POINTER(_c_sz)]),
     "crys_scft_evaluate": (ctypes.c_int, [_c_vp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp,
                                           _c_dp]),
 }

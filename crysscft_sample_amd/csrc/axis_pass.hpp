@@ -225,22 +225,23 @@ struct MfmaShape {
   static_assert(TT % WPJ == 0, "line tiles must split over the waves of a row tile");
 };
 
+// One workgroup's tile `tidx` of a pass on one chain; `tile` is LDS of at
+// least MP * MfmaShape<MP>::LDT elements.  Shared by the single-field kernel
+// and the batched kernel.
 template <typename T, int AXIS, int MP>
-__global__ __launch_bounds__(256) void axis_pass_mfma_kernel(PassParams<T> p) {
+__device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* __restrict__ mat,
+                                               int X, int Y, int Z, int tidx, T* tile) {
   using S = MfmaShape<MP>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int NT = 256;
   constexpr int TL = S::TL, LDT = S::LDT, JT = S::JT, KS = S::KS, WPJ = S::WPJ, TPW = S::TPW;
   constexpr int KT = MP * TL / NT;
   static_assert(MP * TL % NT == 0, "tile must split evenly over the workgroup");
-  __shared__ T tile[MP * LDT];
 
-  const PassChain<T> ch = p.c[blockIdx.y];
-  const int X = p.X, Y = p.Y, Z = p.Z;
   const int M = AXIS == 0 ? X : (AXIS == 1 ? Y : Z);
   const int YZ = Y * Z;
   const int nlines = X * YZ / M;
-  const int l0 = blockIdx.x * TL;
+  const int l0 = tidx * TL;
   const int nl = nlines - l0 < TL ? nlines - l0 : TL;
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(256) void axis_pass_mfma_kernel(PassParams<T> p) {
 #pragma unroll
   for (int kk = 0; kk < KS; ++kk) {
     const int i = 4 * kk + (lane >> 4);
-    a[kk] = (active && i < M && jrow < M) ? p.mat[i * M + jrow] : T(0);
+    a[kk] = (active && i < M && jrow < M) ? mat[i * M + jrow] : T(0);
   }
   T v[KT];
 #pragma unroll
@@ -339,6 +340,12 @@ __global__ __launch_bounds__(256) void axis_pass_mfma_kernel(PassParams<T> p) {
 #pragma unroll
   for (int k = 0; k < KT; ++k)
     if (ok[k]) ch.out[gi[k]] = v[k];
+}
+
+template <typename T, int AXIS, int MP>
+__global__ __launch_bounds__(256) void axis_pass_mfma_kernel(PassParams<T> p) {
+  __shared__ T tile[MP * MfmaShape<MP>::LDT];
+  mfma_pass_tile<T, AXIS, MP>(p.c[blockIdx.y], p.mat, p.X, p.Y, p.Z, blockIdx.x, tile);
 }
 
 // Padded length class for an axis of length M (0 if unsupported).

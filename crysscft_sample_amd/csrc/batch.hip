@@ -1,0 +1,639 @@
+// Many independent SCFT instances advanced together (include/crys_solver.h,
+// crys_batch_*).
+//
+// Each instance is one diblock_scft_simple_iterate (crysSCFT_sample.cpp:426-
+// 531) with its own grid, chemistry, cell and contour.  One launch of
+// batch_pass_kernel runs one axis pass of one contour step for every active
+// instance and both of its chains: blockIdx.x indexes a work list of
+// (instance, chain, tile) items built on the host, ordered by decreasing
+// contour length so that the instances still running at step t are a prefix
+// of the list.  The per-axis length class (16/32/48/64) of each instance
+// selects the matrix-core tile body at run time.  One SCFT iteration of the
+// whole batch is captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "axis_pass.hpp"
+#include "crys_solver.h"
+#include "field_ops.hpp"
+#include "operators.hpp"
+
+namespace crys {
+int set_error(int code, const char* msg);
+}
+
+namespace {
+
+constexpr int kCap = 256;  // trace rows kept on the device between copies
+
+int errf(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  return crys::set_error(code, buf);
+}
+
+#define BHIP(expr)                                                                         \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return errf(e_ == hipErrorOutOfMemory ? CRYS_ERR_OUT_OF_MEMORY : CRYS_ERR_HIP,       \
+                  "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define BCHK(expr)                \
+  do {                            \
+    int r_ = (expr);              \
+    if (r_ != CRYS_OK) return r_; \
+  } while (0)
+
+}  // namespace
+
+namespace crys {
+
+// Device view of one instance.  Pointers are element pointers into the
+// batch's shared allocations.
+template <typename T>
+struct BatchInst {
+  int X, Y, Z, cls[3];
+  int NsA, NsB, Ns, nblk;  // nblk: reduction workgroups of this instance
+  long n;
+  const T* mat[3];
+  T *qf, *qb, *scr0, *scr1, *EA, *EB;
+  double *wA, *wB, *ksi, *phiA, *phiB;
+  double fA, fB, xN, acc, ds;
+};
+
+// LDS elements for the largest class (MP = 64 or 48/32/16).
+constexpr int kBatchLds = 64 * MfmaShape<64>::LDT > 32 * MfmaShape<32>::LDT
+                              ? 64 * MfmaShape<64>::LDT
+                              : 32 * MfmaShape<32>::LDT;
+
+template <typename T, int AXIS>
+__global__ __launch_bounds__(256) void batch_pass_kernel(const BatchInst<T>* __restrict__ insts,
+                                                         const int2* __restrict__ work, int t) {
+  __shared__ T lds[kBatchLds];
+  const int2 w = work[blockIdx.x];
+  const BatchInst<T>& I = insts[w.x];
+  const int chain = w.y >> 16, tidx = w.y & 0xffff;
+  const long n = I.n;
+  T* q = chain == 0 ? I.qf : I.qb;
+  T* scr = chain == 0 ? I.scr0 : I.scr1;
+  const int s_in = chain == 0 ? t : I.Ns - t;
+  const int s_out = chain == 0 ? t + 1 : I.Ns - t - 1;
+  const T* E = chain == 0 ? (t < I.NsA ? I.EA : I.EB) : (s_out >= I.NsA ? I.EB : I.EA);
+  PassChain<T> ch;
+  if constexpr (AXIS == 2) ch = {q + s_in * n, scr, E, nullptr};
+  else if constexpr (AXIS == 1) ch = {scr, scr, nullptr, nullptr};
+  else ch = {scr, q + s_out * n, nullptr, E};
+  const T* mat = I.mat[AXIS];
+  switch (I.cls[AXIS]) {
+    case 16: mfma_pass_tile<T, AXIS, 16>(ch, mat, I.X, I.Y, I.Z, tidx, lds); break;
+    case 32: mfma_pass_tile<T, AXIS, 32>(ch, mat, I.X, I.Y, I.Z, tidx, lds); break;
+    case 48: mfma_pass_tile<T, AXIS, 48>(ch, mat, I.X, I.Y, I.Z, tidx, lds); break;
+    default: mfma_pass_tile<T, AXIS, 64>(ch, mat, I.X, I.Y, I.Z, tidx, lds); break;
+  }
+}
+
+// Q partials: work item (instance, block).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void batch_qsum_kernel(const BatchInst<T>* insts,
+                                                                 const int2* work, double* part) {
+  __shared__ double sh[kRedThreads];
+  const int2 w = work[blockIdx.x];
+  const BatchInst<T>& I = insts[w.x];
+  double acc = 0.0;
+  for (long i = (long)w.y * kRedThreads + threadIdx.x; i < I.n; i += (long)I.nblk * kRedThreads)
+    acc += (double)I.qb[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[(size_t)w.x * kMaxRedBlocks + w.y] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void batch_qfinal_kernel(const BatchInst<T>* insts,
+                                                                   const double* part,
+                                                                   double* scal) {
+  __shared__ double sh[kRedThreads];
+  const BatchInst<T>& I = insts[blockIdx.x];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < I.nblk; i += kRedThreads)
+    acc += part[(size_t)blockIdx.x * kMaxRedBlocks + i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) {
+    const double Q = s / I.n;
+    double* sc = scal + 4 * (size_t)blockIdx.x;
+    sc[0] = Q;
+    sc[1] = I.fA / (I.NsA * Q);
+    sc[2] = I.fB / (I.NsB * Q);
+  }
+}
+
+// integrate_phi, print_energy sums, calc_new_fields and the next calc_ksi /
+// array_exp of one instance (crysSCFT_sample.cpp:348-371, :398-424, :478-480).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void batch_phi_kernel(const BatchInst<T>* insts,
+                                                                const int2* work,
+                                                                const double* scal,
+                                                                double* part) {
+  __shared__ double sh[kRedThreads];
+  const int2 w = work[blockIdx.x];
+  const BatchInst<T>& I = insts[w.x];
+  const double ca = scal[4 * (size_t)w.x + 1], cb = scal[4 * (size_t)w.x + 2];
+  const double index = -I.ds / 2, xN = I.xN, acc_mix = I.acc;
+  const long n = I.n;
+  const int NsA = I.NsA, NsB = I.NsB;
+  double s_e1 = 0.0, s_enth = 0.0, s_fk = 0.0, m_inc = 0.0;
+  for (long i = (long)w.y * kRedThreads + threadIdx.x; i < n; i += (long)I.nblk * kRedThreads) {
+    double pA = 0.0;
+    for (int s = 0; s <= NsA; ++s) {
+      double factor = s == 0 ? 0.5 : (s == NsA ? 0.5 : 1);
+      factor *= ca;
+      pA += factor * (double)I.qf[s * n + i] * (double)I.qb[s * n + i];
+    }
+    double pB = 0.0;
+    for (int s = 0; s <= NsB; ++s) {
+      double factor = s == 0 ? 0.5 : (s == NsB ? 0.5 : 1);
+      factor *= cb;
+      const long k = (NsA + s) * n + i;
+      pB += factor * (double)I.qf[k] * (double)I.qb[k];
+    }
+    const double a = I.wA[i], b = I.wB[i], x = I.ksi[i];
+    s_e1 += -a * pA - b * pB;
+    s_enth += xN * pA * pB;
+    const double tt = 1 - pA - pB;
+    s_fk += -x * tt;
+    m_inc = fabs(tt) > m_inc ? fabs(tt) : m_inc;
+    I.phiA[i] = pA;
+    I.phiB[i] = pB;
+    const double na = a + acc_mix * (xN * pB + x - a);
+    const double nb = b + acc_mix * (xN * pA + x - b);
+    I.wA[i] = na;
+    I.wB[i] = nb;
+    I.ksi[i] = 0.5 * (na + nb - xN);
+    I.EA[i] = (T)(1. * exp(index * na));
+    I.EB[i] = (T)(1. * exp(index * nb));
+  }
+  const double r0 = block_sum(s_e1, sh);
+  const double r1 = block_sum(s_enth, sh);
+  const double r2 = block_sum(s_fk, sh);
+  const double r3 = block_max(m_inc, sh);
+  if (threadIdx.x == 0) {
+    double* p = part + (size_t)w.x * 4 * kMaxRedBlocks;
+    p[0 * kMaxRedBlocks + w.y] = r0;
+    p[1 * kMaxRedBlocks + w.y] = r1;
+    p[2 * kMaxRedBlocks + w.y] = r2;
+    p[3 * kMaxRedBlocks + w.y] = r3;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void batch_efinal_kernel(const BatchInst<T>* insts,
+                                                                   const double* part,
+                                                                   const double* scal,
+                                                                   double* trace,
+                                                                   const int* counter) {
+  __shared__ double sh[kRedThreads];
+  const BatchInst<T>& I = insts[blockIdx.x];
+  const double* p = part + (size_t)blockIdx.x * 4 * kMaxRedBlocks;
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int i = threadIdx.x; i < I.nblk; i += kRedThreads) {
+    a0 += p[0 * kMaxRedBlocks + i];
+    a1 += p[1 * kMaxRedBlocks + i];
+    a2 += p[2 * kMaxRedBlocks + i];
+    a3 = p[3 * kMaxRedBlocks + i] > a3 ? p[3 * kMaxRedBlocks + i] : a3;
+  }
+  const double e1 = block_sum(a0, sh);
+  const double enth = block_sum(a1, sh);
+  const double fk = block_sum(a2, sh);
+  const double mi = block_max(a3, sh);
+  if (threadIdx.x == 0) {
+    const double Q = scal[4 * (size_t)blockIdx.x];
+    const double e0 = -log(Q);
+    const double n = (double)I.n;
+    const double E1 = e1 / n, EN = enth / n, FK = fk / n;
+    double* r = trace + ((size_t)blockIdx.x * kCap + (*counter % kCap)) * 7;
+    r[0] = e0 + E1 + EN + FK;
+    r[1] = EN;
+    r[2] = e0;
+    r[3] = E1;
+    r[4] = FK;
+    r[5] = mi;
+    r[6] = Q;
+  }
+}
+
+__global__ void batch_count_kernel(int* counter, int reset) { *counter = reset ? 0 : *counter + 1; }
+
+}  // namespace crys
+
+struct crys_batch {
+  int precision = CRYS_FP64;
+  size_t count = 0;
+  std::vector<crys_instance_t> desc;
+  std::vector<size_t> n, NsA, NsB, off_f, off_q;  // per instance sizes and offsets
+  std::vector<int> cls;                           // 3 per instance
+  std::vector<size_t> off_m;                      // matrix offsets, 3 per instance
+  hipStream_t stream = nullptr;
+  void *dev = nullptr;  // one allocation for every buffer
+  void* insts = nullptr;
+  int2 *work[3] = {nullptr, nullptr, nullptr}, *rwork = nullptr;
+  std::vector<int> prefix[3];  // work items active at step t, per axis
+  int n_rwork = 0, Ns_max = 0;
+  double *part_q = nullptr, *part_e = nullptr, *scal = nullptr, *trace = nullptr;
+  int* counter = nullptr;
+  std::vector<void*> allocs;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<char> fields_set;
+  // typed base pointers
+  void *qf = nullptr, *qb = nullptr, *scr = nullptr, *EA = nullptr, *EB = nullptr, *mats = nullptr;
+  double *wA = nullptr, *wB = nullptr, *ksi = nullptr, *phiA = nullptr, *phiB = nullptr;
+};
+
+namespace {
+
+int lines_tiles(int mp, size_t nlines) {
+  const int TL = mp <= 32 ? 64 : 32;
+  return (int)((nlines + TL - 1) / TL);
+}
+
+void batch_free(crys_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void* p : b->allocs)
+    if (p) (void)hipFree(p);
+  b->allocs.clear();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->stream = nullptr;
+}
+
+int balloc(crys_batch* b, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  if (e != hipSuccess)
+    return errf(e == hipErrorOutOfMemory ? CRYS_ERR_OUT_OF_MEMORY : CRYS_ERR_HIP,
+                "batch allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  b->allocs.push_back(*p);
+  return CRYS_OK;
+}
+
+template <typename T>
+int batch_setup(crys_batch* b) {
+  const size_t C = b->count;
+  size_t tot_f = 0, tot_q = 0, tot_m = 0;
+  for (size_t i = 0; i < C; ++i) {
+    b->off_f[i] = tot_f;
+    b->off_q[i] = tot_q;
+    tot_f += b->n[i];
+    tot_q += (b->NsA[i] + b->NsB[i] + 1) * b->n[i];
+    const crys_instance_t& d = b->desc[i];
+    for (int a = 0; a < 3; ++a) {
+      b->off_m[3 * i + a] = tot_m;
+      const size_t M = d.kind == CRYS_P1 ? d.grid_log[a] : d.grid_log[a] / 2;
+      tot_m += M * M;
+    }
+  }
+  BCHK(balloc(b, &b->qf, tot_q * sizeof(T)));
+  BCHK(balloc(b, &b->qb, tot_q * sizeof(T)));
+  BCHK(balloc(b, &b->scr, 2 * tot_f * sizeof(T)));
+  BCHK(balloc(b, &b->EA, tot_f * sizeof(T)));
+  BCHK(balloc(b, &b->EB, tot_f * sizeof(T)));
+  BCHK(balloc(b, &b->mats, tot_m * sizeof(T)));
+  double** fd[5] = {&b->wA, &b->wB, &b->ksi, &b->phiA, &b->phiB};
+  for (double** p : fd) BCHK(balloc(b, (void**)p, tot_f * 8));
+  BCHK(balloc(b, (void**)&b->part_q, C * crys::kMaxRedBlocks * 8));
+  BCHK(balloc(b, (void**)&b->part_e, C * 4 * crys::kMaxRedBlocks * 8));
+  BCHK(balloc(b, (void**)&b->scal, C * 4 * 8));
+  BCHK(balloc(b, (void**)&b->trace, C * kCap * 7 * 8));
+  BCHK(balloc(b, (void**)&b->counter, sizeof(int)));
+
+  // operators
+  std::vector<T> mats(tot_m);
+  for (size_t i = 0; i < C; ++i) {
+    const crys_instance_t& d = b->desc[i];
+    for (int a = 0; a < 3; ++a) {
+      std::vector<double> At;
+      const int Nl = (int)d.grid_log[a];
+      // PmmmDCT keeps the reference's cubic cell unless asked not to
+      const double L = (d.kind == CRYS_PMMM_DCT && !(d.flags & CRYS_FLAG_PMMM_ORTHORHOMBIC))
+                           ? d.cell[0]
+                           : d.cell[a];
+      if (d.kind == CRYS_PMMM_DCT) At = crys::pmmm_axis_operator(Nl / 2, L, d.ds);
+      else if (d.kind == CRYS_P1) At = crys::p1_axis_operator(Nl, L, d.ds);
+      else At = crys::m3_axis_operator(Nl, L, d.ds);
+      std::copy(At.begin(), At.end(), mats.begin() + b->off_m[3 * i + a]);
+    }
+  }
+  BHIP(hipMemcpy(b->mats, mats.data(), tot_m * sizeof(T), hipMemcpyHostToDevice));
+
+  // instance table
+  std::vector<crys::BatchInst<T>> tab(C);
+  T* qf = (T*)b->qf;
+  T* qb = (T*)b->qb;
+  T* scr = (T*)b->scr;
+  for (size_t i = 0; i < C; ++i) {
+    const crys_instance_t& d = b->desc[i];
+    crys::BatchInst<T>& I = tab[i];
+    const int red = d.kind == CRYS_P1 ? 1 : 2;
+    I.X = (int)(d.grid_log[0] / red);
+    I.Y = (int)(d.grid_log[1] / red);
+    I.Z = (int)(d.grid_log[2] / red);
+    for (int a = 0; a < 3; ++a) {
+      I.cls[a] = b->cls[3 * i + a];
+      I.mat[a] = (const T*)b->mats + b->off_m[3 * i + a];
+    }
+    I.NsA = (int)b->NsA[i];
+    I.NsB = (int)b->NsB[i];
+    I.Ns = I.NsA + I.NsB;
+    I.n = (long)b->n[i];
+    I.nblk = crys::red_blocks(b->n[i]);
+    I.qf = qf + b->off_q[i];
+    I.qb = qb + b->off_q[i];
+    I.scr0 = scr + b->off_f[i];
+    I.scr1 = scr + (b->off_f[i] + b->off_f.back() + b->n.back());
+    I.EA = (T*)b->EA + b->off_f[i];
+    I.EB = (T*)b->EB + b->off_f[i];
+    I.wA = b->wA + b->off_f[i];
+    I.wB = b->wB + b->off_f[i];
+    I.ksi = b->ksi + b->off_f[i];
+    I.phiA = b->phiA + b->off_f[i];
+    I.phiB = b->phiB + b->off_f[i];
+    I.fA = d.fA;
+    I.fB = 1 - d.fA;
+    I.xN = d.xN;
+    I.acc = d.acceptance;
+    I.ds = d.ds;
+  }
+  BCHK(balloc(b, &b->insts, C * sizeof(crys::BatchInst<T>)));
+  BHIP(hipMemcpy(b->insts, tab.data(), C * sizeof(crys::BatchInst<T>), hipMemcpyHostToDevice));
+
+  // work lists, instances by decreasing contour length
+  std::vector<size_t> order(C);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    return b->NsA[x] + b->NsB[x] > b->NsA[y] + b->NsB[y];
+  });
+  b->Ns_max = (int)(b->NsA[order[0]] + b->NsB[order[0]]);
+  for (int a = 0; a < 3; ++a) {
+    std::vector<int2> wl;
+    std::vector<int> end_of;  // cumulative count after each sorted instance
+    for (size_t k = 0; k < C; ++k) {
+      const size_t i = order[k];
+      const crys::BatchInst<T>& I = tab[i];
+      const int M = a == 0 ? I.X : (a == 1 ? I.Y : I.Z);
+      const int nt = lines_tiles(I.cls[a], (size_t)I.n / M);
+      for (int ch = 0; ch < 2; ++ch)
+        for (int t = 0; t < nt; ++t) wl.push_back(int2{(int)i, (ch << 16) | t});
+      end_of.push_back((int)wl.size());
+    }
+    b->prefix[a].assign(b->Ns_max, 0);
+    for (int t = 0; t < b->Ns_max; ++t) {
+      int cnt = 0;
+      for (size_t k = 0; k < C; ++k)
+        if ((int)(b->NsA[order[k]] + b->NsB[order[k]]) > t) cnt = end_of[k];
+      b->prefix[a][t] = cnt;
+    }
+    BCHK(balloc(b, (void**)&b->work[a], wl.size() * sizeof(int2)));
+    BHIP(hipMemcpy(b->work[a], wl.data(), wl.size() * sizeof(int2), hipMemcpyHostToDevice));
+  }
+  std::vector<int2> rw;
+  for (size_t i = 0; i < C; ++i)
+    for (int k = 0; k < tab[i].nblk; ++k) rw.push_back(int2{(int)i, k});
+  b->n_rwork = (int)rw.size();
+  BCHK(balloc(b, (void**)&b->rwork, rw.size() * sizeof(int2)));
+  BHIP(hipMemcpy(b->rwork, rw.data(), rw.size() * sizeof(int2), hipMemcpyHostToDevice));
+
+  // fixed ends of the chains: q_A(0) = 1 and q_B^dagger(NsB) = 1
+  for (size_t i = 0; i < C; ++i) {
+    const size_t n = b->n[i], Ns = b->NsA[i] + b->NsB[i];
+    hipLaunchKernelGGL((crys::fill_kernel<T>), dim3(256), dim3(256), 0, b->stream, tab[i].qf, (T)1,
+                       n);
+    hipLaunchKernelGGL((crys::fill_kernel<T>), dim3(256), dim3(256), 0, b->stream,
+                       tab[i].qb + Ns * n, (T)1, n);
+  }
+  BHIP(hipGetLastError());
+  BHIP(hipStreamSynchronize(b->stream));
+  return CRYS_OK;
+}
+
+template <typename T>
+int batch_record(crys_batch* b, hipStream_t s) {
+  const crys::BatchInst<T>* I = (const crys::BatchInst<T>*)b->insts;
+  for (int t = 0; t < b->Ns_max; ++t) {
+    const int ax_order[3] = {2, 1, 0};
+    for (int a : ax_order) {
+      const int cnt = b->prefix[a][t];
+      if (cnt == 0) continue;
+      if (a == 2)
+        hipLaunchKernelGGL((crys::batch_pass_kernel<T, 2>), dim3(cnt), dim3(256), 0, s, I,
+                           (const int2*)b->work[2], t);
+      else if (a == 1)
+        hipLaunchKernelGGL((crys::batch_pass_kernel<T, 1>), dim3(cnt), dim3(256), 0, s, I,
+                           (const int2*)b->work[1], t);
+      else
+        hipLaunchKernelGGL((crys::batch_pass_kernel<T, 0>), dim3(cnt), dim3(256), 0, s, I,
+                           (const int2*)b->work[0], t);
+      BHIP(hipGetLastError());
+    }
+  }
+  const unsigned C = (unsigned)b->count;
+  hipLaunchKernelGGL((crys::batch_qsum_kernel<T>), dim3(b->n_rwork), dim3(crys::kRedThreads), 0,
+                     s, I, (const int2*)b->rwork, b->part_q);
+  hipLaunchKernelGGL((crys::batch_qfinal_kernel<T>), dim3(C), dim3(crys::kRedThreads), 0, s, I,
+                     (const double*)b->part_q, b->scal);
+  hipLaunchKernelGGL((crys::batch_phi_kernel<T>), dim3(b->n_rwork), dim3(crys::kRedThreads), 0, s,
+                     I, (const int2*)b->rwork, (const double*)b->scal, b->part_e);
+  hipLaunchKernelGGL((crys::batch_efinal_kernel<T>), dim3(C), dim3(crys::kRedThreads), 0, s, I,
+                     (const double*)b->part_e, (const double*)b->scal, b->trace,
+                     (const int*)b->counter);
+  hipLaunchKernelGGL(crys::batch_count_kernel, dim3(1), dim3(1), 0, s, b->counter, 0);
+  BHIP(hipGetLastError());
+  return CRYS_OK;
+}
+
+int batch_graph(crys_batch* b) {
+  if (b->exec) return CRYS_OK;
+  BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int r = b->precision == CRYS_FP32 ? batch_record<float>(b, b->stream)
+                                    : batch_record<double>(b, b->stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (r != CRYS_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return errf(CRYS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  b->graph = g;
+  BHIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return CRYS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int crys_batch_create(const crys_instance_t* desc, size_t count, int precision,
+                      crys_batch_t** out) {
+  if (!out || !desc || count == 0) return errf(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
+  *out = nullptr;
+  if (precision != CRYS_FP64 && precision != CRYS_FP32)
+    return errf(CRYS_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
+  if (count > 65535) return errf(CRYS_ERR_UNSUPPORTED, "at most 65535 instances");
+  crys_batch* b = new (std::nothrow) crys_batch();
+  if (!b) return errf(CRYS_ERR_OUT_OF_MEMORY, "host allocation failed");
+  b->precision = precision;
+  b->count = count;
+  b->desc.assign(desc, desc + count);
+  b->n.resize(count);
+  b->NsA.resize(count);
+  b->NsB.resize(count);
+  b->off_f.resize(count);
+  b->off_q.resize(count);
+  b->cls.resize(3 * count);
+  b->off_m.resize(3 * count);
+  b->fields_set.assign(count, 0);
+  for (size_t i = 0; i < count; ++i) {
+    const crys_instance_t& d = desc[i];
+    const size_t* g = d.grid_log;
+    if (d.kind != CRYS_P1 && d.kind != CRYS_3M && d.kind != CRYS_PMMM_DCT) {
+      delete b;
+      return errf(CRYS_ERR_INVALID_ARGUMENT, "instance %zu: unknown kind", i);
+    }
+    bool bad = g[0] == 0 || g[1] == 0 || g[2] == 0;
+    if (d.kind == CRYS_P1) bad = bad || g[2] % 8 != 0;
+    else bad = bad || g[0] % 2 || g[1] % 2 || g[2] % 2 || (g[2] / 2) % 8 != 0;
+    if (bad) {
+      delete b;
+      return errf(CRYS_ERR_BAD_GRID, "instance %zu: grid %zu x %zu x %zu rejected", i, g[0], g[1],
+                  g[2]);
+    }
+    if (!(d.ds > 0) || !(d.fA > 0 && d.fA < 1) || !(d.cell[0] > 0 && d.cell[1] > 0 && d.cell[2] > 0)) {
+      delete b;
+      return errf(CRYS_ERR_INVALID_ARGUMENT, "instance %zu: bad ds, fA or cell", i);
+    }
+    b->NsA[i] = (size_t)(d.fA / d.ds);        // crysSCFT_sample.cpp:443
+    b->NsB[i] = (size_t)((1 - d.fA) / d.ds);  // :444
+    if (b->NsA[i] == 0 || b->NsB[i] == 0) {
+      delete b;
+      return errf(CRYS_ERR_INVALID_ARGUMENT, "instance %zu: a block has no contour step", i);
+    }
+    const int red = d.kind == CRYS_P1 ? 1 : 2;
+    b->n[i] = (g[0] / red) * (g[1] / red) * (g[2] / red);
+    for (int a = 0; a < 3; ++a) {
+      const int c = crys::pass_class((int)(g[a] / red));
+      if (c == 0 || c > 64) {
+        delete b;
+        return errf(CRYS_ERR_UNSUPPORTED, "instance %zu: batched axes are limited to 64 points", i);
+      }
+      b->cls[3 * i + a] = c;
+    }
+  }
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete b;
+    return errf(CRYS_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+  }
+  int r = precision == CRYS_FP32 ? batch_setup<float>(b) : batch_setup<double>(b);
+  if (r != CRYS_OK) {
+    batch_free(b);
+    delete b;
+    return r;
+  }
+  *out = b;
+  return CRYS_OK;
+}
+
+int crys_batch_destroy(crys_batch_t* b) {
+  if (!b) return CRYS_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+  delete b;
+  return CRYS_OK;
+}
+
+int crys_batch_set_fields(crys_batch_t* b, size_t i, const double* wA, const double* wB) {
+  if (!b || !wA || !wB || i >= b->count) return errf(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
+  const size_t n = b->n[i], o = b->off_f[i];
+  hipStream_t s = b->stream;
+  BHIP(hipMemcpyAsync(b->wA + o, wA, n * 8, hipMemcpyHostToDevice, s));
+  BHIP(hipMemcpyAsync(b->wB + o, wB, n * 8, hipMemcpyHostToDevice, s));
+  const crys_instance_t& d = b->desc[i];
+  if (b->precision == CRYS_FP32)
+    hipLaunchKernelGGL((crys::prep_fields_kernel<float>), dim3(256), dim3(256), 0, s,
+                       (const double*)b->wA + o, (const double*)b->wB + o, b->ksi + o,
+                       (float*)b->EA + o, (float*)b->EB + o, d.xN, d.ds, n);
+  else
+    hipLaunchKernelGGL((crys::prep_fields_kernel<double>), dim3(256), dim3(256), 0, s,
+                       (const double*)b->wA + o, (const double*)b->wB + o, b->ksi + o,
+                       (double*)b->EA + o, (double*)b->EB + o, d.xN, d.ds, n);
+  BHIP(hipGetLastError());
+  BHIP(hipStreamSynchronize(s));
+  b->fields_set[i] = 1;
+  return CRYS_OK;
+}
+
+int crys_batch_run(crys_batch_t* b, size_t nstep, double* trace) {
+  if (!b) return errf(CRYS_ERR_INVALID_ARGUMENT, "null handle");
+  for (size_t i = 0; i < b->count; ++i)
+    if (!b->fields_set[i]) return errf(CRYS_ERR_INVALID_ARGUMENT, "instance %zu has no fields", i);
+  BCHK(batch_graph(b));
+  hipStream_t s = b->stream;
+  std::vector<double> rows;
+  bool diverged = false;
+  size_t done = 0;
+  while (done < nstep) {
+    const size_t chunk = std::min(nstep - done, (size_t)kCap);
+    hipLaunchKernelGGL(crys::batch_count_kernel, dim3(1), dim3(1), 0, s, b->counter, 1);
+    BHIP(hipGetLastError());
+    for (size_t k = 0; k < chunk; ++k) BHIP(hipGraphLaunch(b->exec, s));
+    if (trace) {
+      rows.resize(b->count * kCap * 7);
+      BHIP(hipMemcpyAsync(rows.data(), b->trace, rows.size() * 8, hipMemcpyDeviceToHost, s));
+    }
+    BHIP(hipStreamSynchronize(s));
+    if (trace)
+      for (size_t i = 0; i < b->count; ++i)
+        for (size_t k = 0; k < chunk; ++k) {
+          const double* r = &rows[(i * kCap + k) * 7];
+          memcpy(trace + (i * nstep + done + k) * 7, r, 7 * 8);
+          if (!std::isfinite(r[0]) || !std::isfinite(r[6])) diverged = true;
+        }
+    done += chunk;
+  }
+  if (diverged) return errf(CRYS_ERR_DIVERGED, "non-finite free energy or Q in the trace");
+  return CRYS_OK;
+}
+
+int crys_batch_get_fields(crys_batch_t* b, size_t i, double* phiA, double* phiB, double* wA,
+                          double* wB) {
+  if (!b || i >= b->count) return errf(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
+  const size_t n = b->n[i], o = b->off_f[i];
+  hipStream_t s = b->stream;
+  if (phiA) BHIP(hipMemcpyAsync(phiA, b->phiA + o, n * 8, hipMemcpyDeviceToHost, s));
+  if (phiB) BHIP(hipMemcpyAsync(phiB, b->phiB + o, n * 8, hipMemcpyDeviceToHost, s));
+  if (wA) BHIP(hipMemcpyAsync(wA, b->wA + o, n * 8, hipMemcpyDeviceToHost, s));
+  if (wB) BHIP(hipMemcpyAsync(wB, b->wB + o, n * 8, hipMemcpyDeviceToHost, s));
+  BHIP(hipStreamSynchronize(s));
+  return CRYS_OK;
+}
+
+int crys_batch_get_info(const crys_batch_t* b, size_t i, size_t* n_phy, size_t* NsA,
+                        size_t* NsB) {
+  if (!b || i >= b->count) return errf(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
+  if (n_phy) *n_phy = b->n[i];
+  if (NsA) *NsA = b->NsA[i];
+  if (NsB) *NsB = b->NsB[i];
+  return CRYS_OK;
+}
+
+}  // extern "C"

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kRedThreads) void qsum_partial_kernel(const T* q0, 
 }
 
 // scal[0] = Q = sum(part)/n, scal[1] = fA/(NsA Q), scal[2] = fB/(NsB Q).
-__global__ __launch_bounds__(kRedThreads) void q_finalize_kernel(const double* part, int nparts,
+static __global__ __launch_bounds__(kRedThreads) void q_finalize_kernel(const double* part, int nparts,
                                                                  double* scal, size_t n, double fA,
                                                                  double fB, size_t NsA,
                                                                  size_t NsB) {
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
 }
 
 // Sums the partials and writes one trace row at row *counter (mod cap).
-__global__ __launch_bounds__(kRedThreads) void energy_finalize_kernel(const double* part,
+static __global__ __launch_bounds__(kRedThreads) void energy_finalize_kernel(const double* part,
                                                                       int nparts,
                                                                       const double* scal,
                                                                       double* trace, int* counter,
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kRedThreads) void dot3_partial_kernel(const T* u, c
 //   H[wA, wB] = (1/n) sum_r [ (wA-wB)^2 / (4 xN) - (wA+wB)/2 + xN/4 ] - ln Q
 // from the densities of the same fields: d(-ln Q)/dwA(r) = cA phiA(r) / n with
 // cA = ds NsA / fA (and B alike).  Partials of the field part go to part.
-__global__ __launch_bounds__(kRedThreads) void hgrad_kernel(const double* wA, const double* wB,
+static __global__ __launch_bounds__(kRedThreads) void hgrad_kernel(const double* wA, const double* wB,
                                                             const double* phiA,
                                                             const double* phiB, double* gA,
                                                             double* gB, double* part, size_t n,
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kRedThreads) void hgrad_kernel(const double* wA, co
 }
 
 // out[k] = sum of part[k * nparts + b] over b, one workgroup per k.
-__global__ __launch_bounds__(kRedThreads) void sum_partials_kernel(const double* part, int nparts,
+static __global__ __launch_bounds__(kRedThreads) void sum_partials_kernel(const double* part, int nparts,
                                                                    double* out) {
   __shared__ double sh[kRedThreads];
   double acc = 0.0;
@@ -243,6 +243,6 @@ __global__ __launch_bounds__(kRedThreads) void sum_partials_kernel(const double*
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-__global__ void set_int_kernel(int* p, int v) { *p = v; }
+static __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 }  // namespace crys

@@ -170,6 +170,41 @@ CRYS_API int crys_diblock_scft_simple_iterate(crys_solver_t* solver, double fA, 
                                      double* trace, double* phiA, double* phiB, double* wA_out,
                                      double* wB_out);
 
+/* ---- many independent SCFT instances ----------------------------------- */
+
+/* One diblock_scft_simple_iterate run (crysSCFT_sample.cpp:426-531): solver
+ * kind (CRYS_3M with zero translational parts), flags (enum crys_flags),
+ * logical grid with every physical axis of at most 64 points, cell lengths,
+ * block fraction fA, xN, contour step ds (NsA = (size_t)(fA/ds),
+ * NsB = (size_t)((1-fA)/ds), :443-444) and mixing acceptance. */
+typedef struct crys_instance {
+  int kind;
+  int flags;
+  size_t grid_log[3];
+  double cell[3];
+  double fA, xN, ds, acceptance;
+} crys_instance_t;
+
+typedef struct crys_batch crys_batch_t;
+
+/* Allocates all instances in HBM and builds their operators.  Every launch
+ * then advances one axis pass of one contour step for all instances whose
+ * contour is not finished, both chains of each; one SCFT iteration of the
+ * whole batch is one hipGraph.  No counterpart in the reference, which runs
+ * one instance per call of diblock_scft_simple_iterate. */
+CRYS_API int crys_batch_create(const crys_instance_t* instances, size_t count, int precision,
+                               crys_batch_t** out);
+CRYS_API int crys_batch_destroy(crys_batch_t* b);
+/* Initial fields of instance i (its physical layout; init_func, :463). */
+CRYS_API int crys_batch_set_fields(crys_batch_t* b, size_t i, const double* wA, const double* wB);
+/* nstep iterations of every instance; trace (may be NULL) receives
+ * count * nstep rows of CRYS_TRACE_COLS doubles, instance-major. */
+CRYS_API int crys_batch_run(crys_batch_t* b, size_t nstep, double* trace);
+CRYS_API int crys_batch_get_fields(crys_batch_t* b, size_t i, double* phiA, double* phiB,
+                                   double* wA, double* wB);
+CRYS_API int crys_batch_get_info(const crys_batch_t* b, size_t i, size_t* n_phy, size_t* NsA,
+                                 size_t* NsB);
+
 /* ---- field files and grid maps (host only) ------------------------------ */
 
 /* output_fields (crysSCFT_sample.cpp:271-282): one line "phiA phiB wA wB" per
