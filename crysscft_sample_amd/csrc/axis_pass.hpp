@@ -194,7 +194,9 @@ __global__ __launch_bounds__(256) void axis_pass_kernel(PassParams<T> p) {
 // A operands (D[j][i] = At[i][j], 16 x 4 per step) stay in registers for the
 // whole pass, the B operands (4 x 16 tile slices) come from LDS.
 // Lane layout (16x16x4, one block): A lane l = A[l%16][l/16],
-// B lane l = B[l/16][l%16], D lane l register r = D[4(l/16)+r][l%16].
+// B lane l = B[l/16][l%16]; the result register r of lane l holds
+// D[4(l/16)+r][l%16] for f32 and D[4r+l/16][l%16] for f64 (checked on gfx950
+// with known operands).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct MfmaOp;
@@ -204,6 +206,7 @@ struct MfmaOp<double> {
   static __device__ inline v4 mma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int r) { return 4 * r + (lane >> 4); }
 };
 template <>
 struct MfmaOp<float> {
@@ -211,6 +214,7 @@ struct MfmaOp<float> {
   static __device__ inline v4 mma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 template <int MP>
@@ -323,7 +327,7 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
     for (int u = 0; u < TPW; ++u) {
       const int col = (sub * TPW + u) * 16 + (lane & 15);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) tile[(jt * 16 + 4 * (lane >> 4) + r) * LDT + col] = acc[u][r];
+      for (int r = 0; r < 4; ++r) tile[(jt * 16 + MfmaOp<T>::row(lane, r)) * LDT + col] = acc[u][r];
     }
   }
   __syncthreads();
