@@ -3,8 +3,10 @@
 Checks, for crys_scft_evaluate:
   - dH/dw against the densities of the oracle (d lnQ / dwA = -(ds NsA / fA)
     phiA / n, the reference's own phi fields) to 1e-10;
-  - dH/dw and dH/dL against central finite differences of the GPU's own H
-    (h = 1e-5) to 1e-6 relative;
+  - dH/dw and dH/dL against fourth-order central finite differences of the
+    GPU's own H to 1e-6 relative (a single-point dH/dw is O(1/n) while H is
+    O(1), so the step must be large enough for H's rounding to stay below
+    1e-6 of the derivative);
   - dH/dL against central finite differences of the oracle's H, and across
     the three solvers on mirror-symmetric fields;
   - the torch.autograd binding against the same numbers.
@@ -44,6 +46,11 @@ def oracle_H(oracle, name, wA, wB, cell):
     return field - np.log(tr[0, 6]), F.extract(phiA, name), F.extract(phiB, name)
 
 
+def central4(f, h):
+    """Fourth-order central difference of f at 0 (truncation O(h^4))."""
+    return (-f(2 * h) + 8 * f(h) - 8 * f(-h) + f(-2 * h)) / (12 * h)
+
+
 def make_run(name, cell=CELL):
     if name == "PmmmDCT":
         s = C.SolverPmmmDCT(N, orthorhombic=True)
@@ -80,15 +87,14 @@ def test_field_gradient_central_differences():
     wA, wB = fields(name)
     s, run = make_run(name)
     g = run.evaluate(wA, wB, stress=False)
-    h = 1e-5
+    h = 2e-3
     for idx in [(0, 0, 0), (3, 9, 14), (15, 1, 7)]:
         for key, base in (("dH_dwA", 0), ("dH_dwB", 1)):
-            hp = []
-            for sgn in (1, -1):
+            def H(d):
                 a, b = wA.copy(), wB.copy()
-                (a if base == 0 else b)[idx] += sgn * h
-                hp.append(run.evaluate(a, b, stress=False)["H"])
-            fd = (hp[0] - hp[1]) / (2 * h)
+                (a if base == 0 else b)[idx] += d
+                return run.evaluate(a, b, stress=False)["H"]
+            fd = central4(H, h)
             an = g[key][idx]
             say(f"  {key}{idx}: fd {fd:.10e} adjoint {an:.10e}")
             assert abs(fd / an - 1) < 1e-6
@@ -99,18 +105,17 @@ def test_stress_central_differences(name):
     wA, wB = fields(name)
     s, run = make_run(name)
     g = run.evaluate(wA, wB, stress=True)
-    h = 1e-5
+    h = 1e-4
     for a in range(3):
-        hp = []
-        for sgn in (1, -1):
+        def H(d):
             cell = list(CELL)
-            cell[a] += sgn * h
+            cell[a] += d
             s.set_cell_para(cell)
             s.set_contour_step(DS)
-            hp.append(run.evaluate(wA, wB, stress=False)["H"])
+            return run.evaluate(wA, wB, stress=False)["H"]
+        fd = central4(H, h)
         s.set_cell_para(CELL)
         s.set_contour_step(DS)
-        fd = (hp[0] - hp[1]) / (2 * h)
         say(f"  {name} dH/dL{a}: fd {fd:.10e} adjoint {g['dH_dL'][a]:.10e}")
         assert abs(fd / g["dH_dL"][a] - 1) < 1e-6
 
@@ -124,14 +129,12 @@ def test_stress_matches_oracle_and_other_solvers(oracle):
     for name in ("3m", "P1"):
         assert rel_err(stress[name], stress["PmmmDCT"]) < 1e-10
     wA, wB = F.extract(wAf, "P1"), F.extract(wBf, "P1")
-    h = 1e-5
     for a in range(3):
-        hp = []
-        for sgn in (1, -1):
+        def H(d):
             cell = list(CELL)
-            cell[a] += sgn * h
-            hp.append(oracle_H(oracle, "P1", wA, wB, cell)[0])
-        fd = (hp[0] - hp[1]) / (2 * h)
+            cell[a] += d
+            return oracle_H(oracle, "P1", wA, wB, cell)[0]
+        fd = central4(H, 1e-4)
         assert abs(fd / stress["P1"][a] - 1) < 1e-6
 
 

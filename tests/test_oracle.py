@@ -175,15 +175,15 @@ def test_adjoint_identity_by_finite_differences(oracle):
     _, _, _, phiA, _ = s.scft(fA, ds, xN, L, 0.0, 1, wA, wB)
     NsA = int(fA / ds)
     n = wA.size
-    h = 1e-4
+    h = 2e-3
     for idx in [(0, 0, 0), (3, 5, 7), (7, 2, 1)]:
-        lq = []
-        for sgn in (+1, -1):
+        def lnQ(d):
             w = wA.copy()
-            w[idx] += sgn * h
+            w[idx] += d
             tr, *_ = s.scft(fA, ds, xN, L, 0.0, 1, w, wB)
-            lq.append(-tr[0, 2])
-        fd = (lq[0] - lq[1]) / (2 * h)
+            return -tr[0, 2]
+        # fourth-order central difference
+        fd = (-lnQ(2 * h) + 8 * lnQ(h) - 8 * lnQ(-h) + lnQ(-2 * h)) / (12 * h)
         an = -(ds * NsA / fA) * phiA[idx] / n
         assert abs(fd / an - 1) < 1e-6
 
