@@ -22,6 +22,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace crys {
 
 template <typename T>
@@ -35,7 +37,8 @@ struct PassChain {
 template <typename T>
 struct PassParams {
   PassChain<T> c[2];
-  const T* mat;  // At, M x M, At[i*M + j]
+  const T* mat;                 // At, M x M, At[i*M + j]
+  const unsigned short* mat_lo; // FP32 only: bf16 bits of At - (float)At, or nullptr
   int X, Y, Z;
 };
 
@@ -211,11 +214,24 @@ struct MfmaOp<double> {
 template <>
 struct MfmaOp<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef short s4 __attribute__((ext_vector_type(4)));
   static __device__ inline v4 mma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // 16x16x16 bf16 product into the same f32 accumulator; lane l holds
+  // A[l%16][4(l/16)+i] and B[4(l/16)+i][l%16] (same result layout as f32).
+  static __device__ inline v4 mma_bf16(s4 a, s4 b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
+
+// float -> bf16 bits, round to nearest even (finite inputs).
+__device__ inline short f32_to_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
 
 template <int MP>
 struct MfmaShape {
@@ -234,7 +250,8 @@ struct MfmaShape {
 // and the batched kernel.
 template <typename T, int AXIS, int MP>
 __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* __restrict__ mat,
-                                               int X, int Y, int Z, int tidx, T* tile) {
+                                               const unsigned short* __restrict__ mat_lo, int X,
+                                               int Y, int Z, int tidx, T* tile) {
   using S = MfmaShape<MP>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int NT = 256;
@@ -294,6 +311,20 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
     const int i = 4 * kk + (lane >> 4);
     a[kk] = (active && i < M && jrow < M) ? mat[i * M + jrow] : T(0);
   }
+  // Split precision (FP32): D = D_hi + D_lo with D_lo in bf16.  The bf16
+  // product of a 16-row chunk c pairs k' = 4(l/16)+i with row 16c+4i+l/16,
+  // the row this lane already holds from f32 step kk = 4c+i, so it reuses the
+  // same A addresses and the same B values.
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  short alo[SPLIT ? KS : 1];
+  const bool use_lo = SPLIT && mat_lo != nullptr;
+  if constexpr (SPLIT) {
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) {
+      const int i = 4 * kk + (lane >> 4);
+      alo[kk] = (use_lo && active && i < M && jrow < M) ? (short)mat_lo[i * M + jrow] : (short)0;
+    }
+  }
   T v[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) v[k] = ok[k] ? ch.in[gi[k]] : T(0);
@@ -315,9 +346,23 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
     if (active) {
       const int col = (sub * TPW + u) * 16 + (lane & 15);
 #pragma unroll
-      for (int kk = 0; kk < KS; ++kk) {
-        const T b = tile[(4 * kk + (lane >> 4)) * LDT + col];
-        acc[u] = MfmaOp<T>::mma(a[kk], b, acc[u]);
+      for (int c4 = 0; c4 < KS / 4; ++c4) {
+        T b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int kk = 4 * c4 + i;
+          b[i] = tile[(4 * kk + (lane >> 4)) * LDT + col];
+          acc[u] = MfmaOp<T>::mma(a[kk], b[i], acc[u]);
+        }
+        if constexpr (SPLIT) {
+          if (use_lo) {
+            typename MfmaOp<float>::s4 av = {alo[4 * c4], alo[4 * c4 + 1], alo[4 * c4 + 2],
+                                             alo[4 * c4 + 3]};
+            typename MfmaOp<float>::s4 bv = {f32_to_bf16(b[0]), f32_to_bf16(b[1]),
+                                             f32_to_bf16(b[2]), f32_to_bf16(b[3])};
+            acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
+          }
+        }
       }
     }
   }
@@ -349,7 +394,7 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
 template <typename T, int AXIS, int MP>
 __global__ __launch_bounds__(256) void axis_pass_mfma_kernel(PassParams<T> p) {
   __shared__ T tile[MP * MfmaShape<MP>::LDT];
-  mfma_pass_tile<T, AXIS, MP>(p.c[blockIdx.y], p.mat, p.X, p.Y, p.Z, blockIdx.x, tile);
+  mfma_pass_tile<T, AXIS, MP>(p.c[blockIdx.y], p.mat, p.mat_lo, p.X, p.Y, p.Z, blockIdx.x, tile);
 }
 
 // Padded length class for an axis of length M (0 if unsupported).

@@ -20,6 +20,7 @@
 #include <new>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "axis_pass.hpp"
@@ -70,6 +71,7 @@ struct BatchInst {
   int NsA, NsB, Ns, nblk;  // nblk: reduction workgroups of this instance
   long n;
   const T* mat[3];
+  const unsigned short* mat_lo[3];  // FP32: bf16 low parts of the operators
   T *qf, *qb, *scr0, *scr1, *EA, *EB;
   double *wA, *wB, *ksi, *phiA, *phiB;
   double fA, fB, xN, acc, ds;
@@ -98,11 +100,12 @@ __global__ __launch_bounds__(256) void batch_pass_kernel(const BatchInst<T>* __r
   else if constexpr (AXIS == 1) ch = {scr, scr, nullptr, nullptr};
   else ch = {scr, q + s_out * n, nullptr, E};
   const T* mat = I.mat[AXIS];
+  const unsigned short* lo = I.mat_lo[AXIS];
   switch (I.cls[AXIS]) {
-    case 16: mfma_pass_tile<T, AXIS, 16>(ch, mat, I.X, I.Y, I.Z, tidx, lds); break;
-    case 32: mfma_pass_tile<T, AXIS, 32>(ch, mat, I.X, I.Y, I.Z, tidx, lds); break;
-    case 48: mfma_pass_tile<T, AXIS, 48>(ch, mat, I.X, I.Y, I.Z, tidx, lds); break;
-    default: mfma_pass_tile<T, AXIS, 64>(ch, mat, I.X, I.Y, I.Z, tidx, lds); break;
+    case 16: mfma_pass_tile<T, AXIS, 16>(ch, mat, lo, I.X, I.Y, I.Z, tidx, lds); break;
+    case 32: mfma_pass_tile<T, AXIS, 32>(ch, mat, lo, I.X, I.Y, I.Z, tidx, lds); break;
+    case 48: mfma_pass_tile<T, AXIS, 48>(ch, mat, lo, I.X, I.Y, I.Z, tidx, lds); break;
+    default: mfma_pass_tile<T, AXIS, 64>(ch, mat, lo, I.X, I.Y, I.Z, tidx, lds); break;
   }
 }
 
@@ -258,6 +261,7 @@ struct crys_batch {
   std::vector<char> fields_set;
   // typed base pointers
   void *qf = nullptr, *qb = nullptr, *scr = nullptr, *EA = nullptr, *EB = nullptr, *mats = nullptr;
+  unsigned short* mats_lo = nullptr;  // FP32: bf16 low parts of the operators
   double *wA = nullptr, *wB = nullptr, *ksi = nullptr, *phiA = nullptr, *phiB = nullptr;
 };
 
@@ -319,6 +323,7 @@ int batch_setup(crys_batch* b) {
 
   // operators
   std::vector<T> mats(tot_m);
+  std::vector<unsigned short> lo(std::is_same<T, float>::value ? tot_m : 0);
   for (size_t i = 0; i < C; ++i) {
     const crys_instance_t& d = b->desc[i];
     for (int a = 0; a < 3; ++a) {
@@ -332,9 +337,17 @@ int batch_setup(crys_batch* b) {
       else if (d.kind == CRYS_P1) At = crys::p1_axis_operator(Nl, L, d.ds);
       else At = crys::m3_axis_operator(Nl, L, d.ds);
       std::copy(At.begin(), At.end(), mats.begin() + b->off_m[3 * i + a]);
+      if (std::is_same<T, float>::value)
+        for (size_t k = 0; k < At.size(); ++k)
+          lo[b->off_m[3 * i + a] + k] =
+              crys::bf16_bits((float)(At[k] - (double)(float)At[k]));
     }
   }
   BHIP(hipMemcpy(b->mats, mats.data(), tot_m * sizeof(T), hipMemcpyHostToDevice));
+  if (std::is_same<T, float>::value) {
+    BCHK(balloc(b, (void**)&b->mats_lo, tot_m * sizeof(unsigned short)));
+    BHIP(hipMemcpy(b->mats_lo, lo.data(), tot_m * sizeof(unsigned short), hipMemcpyHostToDevice));
+  }
 
   // instance table
   std::vector<crys::BatchInst<T>> tab(C);
@@ -351,6 +364,7 @@ int batch_setup(crys_batch* b) {
     for (int a = 0; a < 3; ++a) {
       I.cls[a] = b->cls[3 * i + a];
       I.mat[a] = (const T*)b->mats + b->off_m[3 * i + a];
+      I.mat_lo[a] = b->mats_lo ? b->mats_lo + b->off_m[3 * i + a] : nullptr;
     }
     I.NsA = (int)b->NsA[i];
     I.NsB = (int)b->NsB[i];

@@ -76,6 +76,7 @@ PassImpl pass_impl() {
 struct OpSet {
   double ds;
   void* dmat[3];     // device, element type of the solver's precision
+  void* dmat_lo[3];  // FP32: bf16 bits of D - (float)D (split-precision product)
   void* dmat_dL[3];  // dD/dL per axis, built on first use (cell stress)
 };
 
@@ -117,13 +118,17 @@ void free_ops(crys_solver* h) {
   for (auto& o : h->ops)
     for (int a = 0; a < 3; ++a) {
       if (o.dmat[a]) (void)hipFree(o.dmat[a]);
+      if (o.dmat_lo[a]) (void)hipFree(o.dmat_lo[a]);
       if (o.dmat_dL[a]) (void)hipFree(o.dmat_dL[a]);
     }
   h->ops.clear();
   h->cur = -1;
 }
 
-int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3]) {
+// Builds and uploads the three per-axis operators (or their L derivatives).
+// For FP32 solvers `lo` (if not null) receives the bf16 bits of
+// D - (float)D, the low part of the split-precision product.
+int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3] = nullptr) {
   for (int a = 0; a < 3; ++a) {
     std::vector<double> At;
     const int M = (int)h->nphy[a];
@@ -135,6 +140,12 @@ int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3]) {
     if (h->precision == CRYS_FP32) {
       std::vector<float> f(At.begin(), At.end());
       HIPCHK(hipMemcpy(dst[a], f.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+      if (lo) {
+        std::vector<unsigned short> b(cnt);
+        for (size_t i = 0; i < cnt; ++i) b[i] = crys::bf16_bits((float)(At[i] - (double)f[i]));
+        HIPCHK(hipMalloc(&lo[a], cnt * sizeof(unsigned short)));
+        HIPCHK(hipMemcpy(lo[a], b.data(), cnt * sizeof(unsigned short), hipMemcpyHostToDevice));
+      }
     } else {
       HIPCHK(hipMemcpy(dst[a], At.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
     }
@@ -144,8 +155,8 @@ int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3]) {
 
 int build_ops(crys_solver* h, double ds, OpSet* out) {
   out->ds = ds;
-  for (int a = 0; a < 3; ++a) out->dmat[a] = out->dmat_dL[a] = nullptr;
-  return upload_axis_ops(h, ds, false, out->dmat);
+  for (int a = 0; a < 3; ++a) out->dmat[a] = out->dmat_lo[a] = out->dmat_dL[a] = nullptr;
+  return upload_axis_ops(h, ds, false, out->dmat, out->dmat_lo);
 }
 
 // One contour step E D E (or a bare D with E == nullptr) for 1 or 2 chains:
@@ -159,12 +170,15 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
   p.Z = (int)h->nphy[2];
   for (int c = 0; c < nch; ++c) p.c[c] = {in[c], scr[c], E[c], nullptr};
   p.mat = (const T*)op.dmat[2];
+  p.mat_lo = (const unsigned short*)op.dmat_lo[2];
   HIPCHK(crys::launch_axis_pass<T>(2, p, nch, s));
   for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], scr[c], nullptr, nullptr};
   p.mat = (const T*)op.dmat[1];
+  p.mat_lo = (const unsigned short*)op.dmat_lo[1];
   HIPCHK(crys::launch_axis_pass<T>(1, p, nch, s));
   for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], out[c], nullptr, E[c]};
   p.mat = (const T*)op.dmat[0];
+  p.mat_lo = (const unsigned short*)op.dmat_lo[0];
   HIPCHK(crys::launch_axis_pass<T>(0, p, nch, s));
   return CRYS_OK;
 }
@@ -341,8 +355,10 @@ int crys_set_contour_step(crys_solver_t* h, double ds) {
   OpSet o;
   int r = build_ops(h, ds, &o);
   if (r != CRYS_OK) {
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
       if (o.dmat[a]) (void)hipFree(o.dmat[a]);
+      if (o.dmat_lo[a]) (void)hipFree(o.dmat_lo[a]);
+    }
     return r;
   }
   h->ops.push_back(o);
@@ -554,6 +570,7 @@ int scft_record_stress(crys_scft* h, hipStream_t s) {
         const int ax = order[q];
         p.c[0] = {q == 0 ? qf + t * n : w, w, q == 0 ? E : nullptr, nullptr};
         p.mat = (const T*)(ax == a ? op.dmat_dL[ax] : op.dmat[ax]);
+        p.mat_lo = ax == a ? nullptr : (const unsigned short*)op.dmat_lo[ax];
         HIPCHK(crys::launch_axis_pass<T>(ax, p, 1, s));
       }
       hipLaunchKernelGGL((crys::dot3_partial_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,

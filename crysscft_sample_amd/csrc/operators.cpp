@@ -5,6 +5,7 @@
 #include "operators.hpp"
 
 #include <cmath>
+#include <cstring>
 
 namespace crys {
 
@@ -84,6 +85,13 @@ std::vector<double> m3_axis_operator(int N, double L, double ds, bool dL) {
       At[(size_t)i * M + j] = (double)v;
     }
   return At;
+}
+
+unsigned short bf16_bits(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, sizeof(u));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
 }
 
 }  // namespace crys

@@ -32,4 +32,7 @@ std::vector<double> pmmm_axis_operator(int M, double L, double ds, bool dL = fal
 std::vector<double> p1_axis_operator(int N, double L, double ds, bool dL = false);
 std::vector<double> m3_axis_operator(int N_logical, double L, double ds, bool dL = false);
 
+// bf16 bits of x, round to nearest even (finite x).
+unsigned short bf16_bits(float x);
+
 }  // namespace crys
