@@ -412,6 +412,194 @@ inline int pass_class(int M) {
 enum class PassImpl { Mfma, Valu };
 PassImpl pass_impl();
 
+// ---------------------------------------------------------------------------
+// Fused z + y passes on x-slabs.  One workgroup (512 threads, 8 waves) owns
+// the Y x Z plane of one x index: it stages the plane in LDS transposed
+// (z-major), applies Dz along z and Dy along y with the matrix cores, and
+// stores the plane back.  With the x pass that follows, a contour step takes
+// two launches instead of three.  Requires Y and Z in the same length class
+// MP <= 64.
+// ---------------------------------------------------------------------------
+template <int MP>
+struct SlabShape {
+  static constexpr int NT = 512;
+  static constexpr int P = MP + 4;                 // LDS row pitch (elements)
+  static constexpr int JT = MP / 16;               // 16-row output tiles
+  static constexpr int CT = MP / 16;               // 16-column output tiles
+  static constexpr int KS = MP / 4;                // MFMA k steps
+  static constexpr int NTILE = JT * CT;            // output tiles per GEMM
+  static constexpr int TPW = (NTILE + 7) / 8;      // tiles per wave
+};
+
+// One GEMM of the slab: out[c][j] = sum_k A[j][k] * src[k][c] for the 16 x 16
+// tiles of wave w, with src stored as lds[k * P + c]; results are returned in
+// registers (acc) for the caller to write once every wave has read src.
+template <typename T, int MP>
+__device__ __forceinline__ void slab_gemm(const T* __restrict__ mat,
+                                          const unsigned short* __restrict__ mat_lo, int M,
+                                          const T* lds, int w, int lane,
+                                          typename MfmaOp<T>::v4 (&acc)[SlabShape<MP>::TPW]) {
+  using S = SlabShape<MP>;
+  using V4 = typename MfmaOp<T>::v4;
+  constexpr int P = S::P, JT = S::JT, KS = S::KS, NTILE = S::NTILE, TPW = S::TPW;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  const bool use_lo = SPLIT && mat_lo != nullptr;
+  int cur_jt = -1;
+  T a[KS];
+  short alo[SPLIT ? KS : 1];
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    acc[u] = V4{T(0), T(0), T(0), T(0)};
+    const int q = w + 8 * u;  // tile index: jt = q % JT, ct = q / JT
+    if (q >= NTILE) continue;
+    const int jt = q % JT, ct = q / JT;
+    if (jt != cur_jt) {
+      cur_jt = jt;
+      const int jrow = jt * 16 + (lane & 15);
+#pragma unroll
+      for (int kk = 0; kk < KS; ++kk) {
+        const int i = 4 * kk + (lane >> 4);
+        a[kk] = (i < M && jrow < M) ? mat[i * M + jrow] : T(0);
+        if constexpr (SPLIT)
+          alo[kk] = (use_lo && i < M && jrow < M) ? (short)mat_lo[i * M + jrow] : (short)0;
+      }
+    }
+    const int col = ct * 16 + (lane & 15);
+#pragma unroll
+    for (int c4 = 0; c4 < KS / 4; ++c4) {
+      T b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int kk = 4 * c4 + i;
+        b[i] = lds[(4 * kk + (lane >> 4)) * P + col];
+        acc[u] = MfmaOp<T>::mma(a[kk], b[i], acc[u]);
+      }
+      if constexpr (SPLIT) {
+        if (use_lo) {
+          typename MfmaOp<float>::s4 av = {alo[4 * c4], alo[4 * c4 + 1], alo[4 * c4 + 2],
+                                           alo[4 * c4 + 3]};
+          typename MfmaOp<float>::s4 bv = {f32_to_bf16(b[0]), f32_to_bf16(b[1]),
+                                           f32_to_bf16(b[2]), f32_to_bf16(b[3])};
+          acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+struct SlabParams {
+  PassChain<T> c[2];
+  const T* mat_z;
+  const T* mat_y;
+  const unsigned short* lo_z;
+  const unsigned short* lo_y;
+  int X, Y, Z;
+};
+
+template <typename T, int MP>
+__global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
+  using S = SlabShape<MP>;
+  constexpr int NT = S::NT, P = S::P, JT = S::JT, TPW = S::TPW, NTILE = S::NTILE;
+  constexpr int KE = (MP * MP + NT - 1) / NT;  // plane elements per thread
+  __shared__ T lds[MP * P];
+  using V4 = typename MfmaOp<T>::v4;
+
+  const PassChain<T> ch = p.c[blockIdx.y];
+  const int Y = p.Y, Z = p.Z;
+  const long base = (long)blockIdx.x * Y * Z;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+
+  // ---- stage the plane transposed: lds[z * P + y] (zero padding) ----
+  T v[KE];
+  int gy[KE], gz[KE];
+  bool ok[KE];
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    const int f = tid + NT * k;
+    const int y = f / MP, z = f - (f / MP) * MP;
+    gy[k] = y;
+    gz[k] = z;
+    ok[k] = f < MP * MP && y < Y && z < Z;
+    v[k] = ok[k] ? ch.in[base + (long)y * Z + z] : T(0);
+  }
+  if (ch.pre) {
+    T e[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) e[k] = ok[k] ? ch.pre[base + (long)gy[k] * Z + gz[k]] : T(0);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) v[k] *= e[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KE; ++k)
+    if (tid + NT * k < MP * MP) lds[gz[k] * P + gy[k]] = v[k];
+  __syncthreads();
+
+  // ---- z pass: t[y][j] = sum_z Dz[j][z] * in[y][z], src lds[z][y] ----
+  V4 acc[TPW];
+  slab_gemm<T, MP>(p.mat_z, p.lo_z, Z, lds, w, lane, acc);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    const int q = w + 8 * u;
+    if (q < NTILE) {
+      const int jt = q % JT, ct = q / JT;
+      const int col = ct * 16 + (lane & 15);  // y
+#pragma unroll
+      for (int r = 0; r < 4; ++r) lds[col * P + jt * 16 + MfmaOp<T>::row(lane, r)] = acc[u][r];
+    }
+  }
+  __syncthreads();
+
+  // ---- y pass: o[j][z] = sum_y Dy[j][y] * t[y][z], src lds[y][z] ----
+  slab_gemm<T, MP>(p.mat_y, p.lo_y, Y, lds, w, lane, acc);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    const int q = w + 8 * u;
+    if (q < NTILE) {
+      const int jt = q % JT, ct = q / JT;
+      const int col = ct * 16 + (lane & 15);  // z
+#pragma unroll
+      for (int r = 0; r < 4; ++r) lds[(jt * 16 + MfmaOp<T>::row(lane, r)) * P + col] = acc[u][r];
+    }
+  }
+  __syncthreads();
+
+  // ---- store the plane (row-major y, z), optional post multiply ----
+#pragma unroll
+  for (int k = 0; k < KE; ++k) v[k] = lds[gy[k] * P + gz[k]];
+  if (ch.post) {
+    T e[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) e[k] = ok[k] ? ch.post[base + (long)gy[k] * Z + gz[k]] : T(0);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) v[k] *= e[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KE; ++k)
+    if (ok[k]) ch.out[base + (long)gy[k] * Z + gz[k]] = v[k];
+}
+
+bool slab_enabled();
+
+// Launches the fused z + y pass when Y and Z share a class <= 64; returns
+// hipErrorNotSupported otherwise (the caller then runs the two passes).
+template <typename T>
+hipError_t launch_slab_zy(const SlabParams<T>& p, int nchains, hipStream_t s) {
+  const int cy = pass_class(p.Y), cz = pass_class(p.Z);
+  if (!slab_enabled() || cy != cz || cy > 64 || cy == 0 || nchains < 1 || nchains > 2)
+    return hipErrorNotSupported;
+  dim3 grid((unsigned)p.X, (unsigned)nchains, 1);
+  switch (cy) {
+    case 16: hipLaunchKernelGGL((slab_zy_kernel<T, 16>), grid, dim3(512), 0, s, p); break;
+    case 32: hipLaunchKernelGGL((slab_zy_kernel<T, 32>), grid, dim3(512), 0, s, p); break;
+    case 48: hipLaunchKernelGGL((slab_zy_kernel<T, 48>), grid, dim3(512), 0, s, p); break;
+    default: hipLaunchKernelGGL((slab_zy_kernel<T, 64>), grid, dim3(512), 0, s, p); break;
+  }
+  return hipGetLastError();
+}
+
 template <typename T, int AXIS, int MP>
 hipError_t launch_axis_pass_mp(const PassParams<T>& p, int nchains, hipStream_t s) {
   const int M = AXIS == 0 ? p.X : (AXIS == 1 ? p.Y : p.Z);

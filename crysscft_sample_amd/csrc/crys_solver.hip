@@ -61,6 +61,15 @@ unsigned grid_for(size_t n) {
 namespace crys {
 int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
 
+// CRYS_SLAB=0 runs the z and y passes as two launches (for A/B timing).
+bool slab_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("CRYS_SLAB");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
 // CRYS_AXIS_PASS=valu selects the FMA pass kernel (for A/B timing); the
 // matrix-core kernel is the default.
 PassImpl pass_impl() {
@@ -164,18 +173,34 @@ int build_ops(crys_solver* h, double ds, OpSet* out) {
 template <typename T>
 int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const in[2],
                 T* const out[2], T* const scr[2], const T* const E[2], hipStream_t s) {
+  const int X = (int)h->nphy[0], Y = (int)h->nphy[1], Z = (int)h->nphy[2];
   crys::PassParams<T> p{};
-  p.X = (int)h->nphy[0];
-  p.Y = (int)h->nphy[1];
-  p.Z = (int)h->nphy[2];
-  for (int c = 0; c < nch; ++c) p.c[c] = {in[c], scr[c], E[c], nullptr};
-  p.mat = (const T*)op.dmat[2];
-  p.mat_lo = (const unsigned short*)op.dmat_lo[2];
-  HIPCHK(crys::launch_axis_pass<T>(2, p, nch, s));
-  for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], scr[c], nullptr, nullptr};
-  p.mat = (const T*)op.dmat[1];
-  p.mat_lo = (const unsigned short*)op.dmat_lo[1];
-  HIPCHK(crys::launch_axis_pass<T>(1, p, nch, s));
+  p.X = X;
+  p.Y = Y;
+  p.Z = Z;
+  // z and y fused on x-slabs when the two axes share a length class
+  crys::SlabParams<T> sp{};
+  sp.X = X;
+  sp.Y = Y;
+  sp.Z = Z;
+  for (int c = 0; c < nch; ++c) sp.c[c] = {in[c], scr[c], E[c], nullptr};
+  sp.mat_z = (const T*)op.dmat[2];
+  sp.mat_y = (const T*)op.dmat[1];
+  sp.lo_z = (const unsigned short*)op.dmat_lo[2];
+  sp.lo_y = (const unsigned short*)op.dmat_lo[1];
+  hipError_t e = crys::launch_slab_zy<T>(sp, nch, s);
+  if (e == hipErrorNotSupported) {
+    for (int c = 0; c < nch; ++c) p.c[c] = {in[c], scr[c], E[c], nullptr};
+    p.mat = (const T*)op.dmat[2];
+    p.mat_lo = (const unsigned short*)op.dmat_lo[2];
+    HIPCHK(crys::launch_axis_pass<T>(2, p, nch, s));
+    for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], scr[c], nullptr, nullptr};
+    p.mat = (const T*)op.dmat[1];
+    p.mat_lo = (const unsigned short*)op.dmat_lo[1];
+    HIPCHK(crys::launch_axis_pass<T>(1, p, nch, s));
+  } else {
+    HIPCHK(e);
+  }
   for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], out[c], nullptr, E[c]};
   p.mat = (const T*)op.dmat[0];
   p.mat_lo = (const unsigned short*)op.dmat_lo[0];
