@@ -40,6 +40,7 @@ struct PassParams {
   const T* mat;                 // At, M x M, At[i*M + j]
   const unsigned short* mat_lo; // FP32 only: bf16 bits of At - (float)At, or nullptr
   int X, Y, Z;
+  int cen;                      // 1: mat is the half-size centrosymmetric operator
 };
 
 // Register micro-tile width along the lines for a padded length MP.
@@ -233,6 +234,19 @@ __device__ inline short f32_to_bf16(float x) {
   return (short)(u >> 16);
 }
 
+// Far-first order of the KS k groups for a tile whose diagonal sits at group
+// position gc (two pointers from the ends inwards).
+template <int KS>
+__device__ __forceinline__ void far_first_order(float gc, int (&ord)[KS]) {
+  int lo = 0, hi = KS - 1;
+#pragma unroll
+  for (int n = 0; n < KS; ++n) {
+    const float dl = fabsf((float)lo - gc), dh = fabsf((float)hi - gc);
+    if (dl >= dh) ord[n] = lo++;
+    else ord[n] = hi--;
+  }
+}
+
 template <int MP>
 struct MfmaShape {
   static constexpr int TL = MP <= 32 ? 64 : 32;  // lines per workgroup
@@ -247,17 +261,24 @@ struct MfmaShape {
 
 // One workgroup's tile `tidx` of a pass on one chain; `tile` is LDS of at
 // least MP * MfmaShape<MP>::LDT elements.  Shared by the single-field kernel
-// and the batched kernel.
-template <typename T, int AXIS, int MP>
+// and the batched kernel.  With CEN (centrosymmetric operator, M == MP,
+// MP % 32 == 0) the lines are split into u/v halves and `mat` is the
+// half-size operator At_c (M/2 x M) described at SlabShape.  The k groups of
+// every output tile are summed far-first (see SlabShape).
+template <typename T, int AXIS, int MP, bool CEN = false>
 __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* __restrict__ mat,
                                                const unsigned short* __restrict__ mat_lo, int X,
                                                int Y, int Z, int tidx, T* tile) {
   using S = MfmaShape<MP>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int NT = 256;
-  constexpr int TL = S::TL, LDT = S::LDT, JT = S::JT, KS = S::KS, WPJ = S::WPJ, TPW = S::TPW;
-  constexpr int KT = MP * TL / NT;
-  static_assert(MP * TL % NT == 0, "tile must split evenly over the workgroup");
+  constexpr int TL = S::TL, LDT = S::LDT, JT = S::JT, WPJ = S::WPJ, TPW = S::TPW;
+  constexpr int KS = CEN ? MP / 8 : MP / 4;
+  constexpr int H = MP / 2;
+  constexpr int ROWS = CEN ? H : MP;          // staged rows (pairs for CEN)
+  constexpr int KT = (ROWS * TL + NT - 1) / NT;  // slots per thread
+  static_assert(!CEN || MP % 32 == 0, "centrosymmetric split needs M % 32 == 0");
+  constexpr bool SPLIT = std::is_same<T, float>::value;
 
   const int M = AXIS == 0 ? X : (AXIS == 1 ? Y : Z);
   const int YZ = Y * Z;
@@ -270,96 +291,133 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
   const int x0 = yplane ? l0 / Z : 0;
   const int z0 = yplane ? l0 - x0 * Z : 0;
 
-  int gi[KT], lo[KT];
+  // global index of element (row i, line ll)
+  auto gidx = [&](int i, int ll) -> int {
+    if constexpr (AXIS == 2) {
+      return (l0 + ll) * Z + i;
+    } else if constexpr (AXIS == 1) {
+      if (yplane) return x0 * YZ + i * Z + z0 + ll;
+      const int l = l0 + ll, x = l / Z;
+      return x * YZ + i * Z + (l - x * Z);
+    } else {
+      return i * YZ + l0 + ll;
+    }
+  };
+
+  // ---- slot k of this thread: row i (< ROWS), line ll ----
+  int si[KT], sl[KT];
   bool ok[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int f = tid + NT * k;
     int i, ll;
     if constexpr (AXIS == 2) {
-      ll = f / MP;
-      i = f - ll * MP;
+      ll = f / ROWS;
+      i = f - ll * ROWS;
     } else {
       i = f / TL;
       ll = f - i * TL;
     }
-    ok[k] = i < M && ll < nl;
-    lo[k] = i * LDT + ll;
-    int g;
-    if constexpr (AXIS == 2) {
-      g = (l0 + ll) * Z + i;
-    } else if constexpr (AXIS == 1) {
-      if (yplane) {
-        g = x0 * YZ + i * Z + z0 + ll;
-      } else {
-        const int l = l0 + ll, x = l / Z;
-        g = x * YZ + i * Z + (l - x * Z);
-      }
-    } else {
-      g = i * YZ + l0 + ll;
-    }
-    gi[k] = ok[k] ? g : 0;
+    si[k] = i;
+    sl[k] = ll;
+    ok[k] = f < ROWS * TL && i < (CEN ? H : M) && ll < nl;
   }
 
-  // operands of this wave: rows jt*16.. of D, all k steps
+  // operands of this wave: row tile jt, k groups far-first
   const int jt = w % JT, sub = w / JT;
   const bool active = w < JT * WPJ;
   const int jrow = jt * 16 + (lane & 15);
+  const int jb = CEN ? (jt % (JT / 2)) : jt;
+  const int kbase = CEN ? (jt < JT / 2 ? 0 : H) : 0;
+  int ord[KS];
+  far_first_order<KS>(4.0f * jb + 1.5f, ord);
+  const int ldm = CEN ? MP : M;
   T a[KS];
-#pragma unroll
-  for (int kk = 0; kk < KS; ++kk) {
-    const int i = 4 * kk + (lane >> 4);
-    a[kk] = (active && i < M && jrow < M) ? mat[i * M + jrow] : T(0);
-  }
-  // Split precision (FP32): D = D_hi + D_lo with D_lo in bf16.  The bf16
-  // product of a 16-row chunk c pairs k' = 4(l/16)+i with row 16c+4i+l/16,
-  // the row this lane already holds from f32 step kk = 4c+i, so it reuses the
-  // same A addresses and the same B values.
-  constexpr bool SPLIT = std::is_same<T, float>::value;
   short alo[SPLIT ? KS : 1];
-  const bool use_lo = SPLIT && mat_lo != nullptr;
-  if constexpr (SPLIT) {
+  int brow[KS];
 #pragma unroll
-    for (int kk = 0; kk < KS; ++kk) {
-      const int i = 4 * kk + (lane >> 4);
-      alo[kk] = (use_lo && active && i < M && jrow < M) ? (short)mat_lo[i * M + jrow] : (short)0;
+  for (int n = 0; n < KS; ++n) {
+    const int i = 4 * ord[n] + (lane >> 4);
+    brow[n] = kbase + i;
+    const bool okm = active && (CEN || (i < M && jrow < M));
+    a[n] = okm ? mat[i * ldm + jrow] : T(0);
+    if constexpr (SPLIT) alo[n] = (okm && mat_lo) ? (short)mat_lo[i * ldm + jrow] : (short)0;
+  }
+
+  // ---- stage (issue every global load first) ----
+  if constexpr (!CEN) {
+    T v[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] = ok[k] ? ch.in[gidx(si[k], sl[k])] : T(0);
+    if (ch.pre) {
+      T e[KT];
+#pragma unroll
+      for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.pre[gidx(si[k], sl[k])] : T(0);
+#pragma unroll
+      for (int k = 0; k < KT; ++k) v[k] *= e[k];
     }
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (tid + NT * k < ROWS * TL) tile[si[k] * LDT + sl[k]] = v[k];
+  } else {
+    T x1[KT], x2[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      x1[k] = ok[k] ? ch.in[gidx(si[k], sl[k])] : T(0);
+      x2[k] = ok[k] ? ch.in[gidx(MP - 1 - si[k], sl[k])] : T(0);
+    }
+    if (ch.pre) {
+      T e1[KT], e2[KT];
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        e1[k] = ok[k] ? ch.pre[gidx(si[k], sl[k])] : T(0);
+        e2[k] = ok[k] ? ch.pre[gidx(MP - 1 - si[k], sl[k])] : T(0);
+      }
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        x1[k] *= e1[k];
+        x2[k] *= e2[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (tid + NT * k < ROWS * TL) {
+        tile[si[k] * LDT + sl[k]] = x1[k] + x2[k];
+        tile[(H + si[k]) * LDT + sl[k]] = x1[k] - x2[k];
+      }
   }
-  T v[KT];
-#pragma unroll
-  for (int k = 0; k < KT; ++k) v[k] = ok[k] ? ch.in[gi[k]] : T(0);
-  if (ch.pre) {
-    T e[KT];
-#pragma unroll
-    for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.pre[gi[k]] : T(0);
-#pragma unroll
-    for (int k = 0; k < KT; ++k) v[k] *= e[k];
-  }
-#pragma unroll
-  for (int k = 0; k < KT; ++k) tile[lo[k]] = v[k];
   __syncthreads();
 
+  // ---- out[j][t] = sum_k A[j][k] * tile[k][t] ----
   V4 acc[TPW];
+  const bool use_lo = SPLIT && mat_lo != nullptr;
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
     acc[u] = V4{T(0), T(0), T(0), T(0)};
     if (active) {
       const int col = (sub * TPW + u) * 16 + (lane & 15);
 #pragma unroll
-      for (int c4 = 0; c4 < KS / 4; ++c4) {
+      for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
         T b[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int kk = 4 * c4 + i;
-          b[i] = tile[(4 * kk + (lane >> 4)) * LDT + col];
-          acc[u] = MfmaOp<T>::mma(a[kk], b[i], acc[u]);
+          const int n = 4 * c4 + i;
+          if (n < KS) {
+            b[i] = tile[brow[n] * LDT + col];
+            acc[u] = MfmaOp<T>::mma(a[n], b[i], acc[u]);
+          } else {
+            b[i] = T(0);
+          }
         }
         if constexpr (SPLIT) {
           if (use_lo) {
-            typename MfmaOp<float>::s4 av = {alo[4 * c4], alo[4 * c4 + 1], alo[4 * c4 + 2],
-                                             alo[4 * c4 + 3]};
-            typename MfmaOp<float>::s4 bv = {f32_to_bf16(b[0]), f32_to_bf16(b[1]),
-                                             f32_to_bf16(b[2]), f32_to_bf16(b[3])};
+            typename MfmaOp<float>::s4 av, bv;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int n = 4 * c4 + i;
+              av[i] = n < KS ? alo[n] : (short)0;
+              bv[i] = f32_to_bf16(b[i]);
+            }
             acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
           }
         }
@@ -377,24 +435,57 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
   }
   __syncthreads();
 
+  // ---- coalesced store, with the optional post multiply ----
+  if constexpr (!CEN) {
+    T v[KT];
 #pragma unroll
-  for (int k = 0; k < KT; ++k) v[k] = tile[lo[k]];
-  if (ch.post) {
-    T e[KT];
+    for (int k = 0; k < KT; ++k) v[k] = ok[k] ? tile[si[k] * LDT + sl[k]] : T(0);
+    if (ch.post) {
+      T e[KT];
 #pragma unroll
-    for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.post[gi[k]] : T(0);
+      for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.post[gidx(si[k], sl[k])] : T(0);
 #pragma unroll
-    for (int k = 0; k < KT; ++k) v[k] *= e[k];
+      for (int k = 0; k < KT; ++k) v[k] *= e[k];
+    }
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (ok[k]) ch.out[gidx(si[k], sl[k])] = v[k];
+  } else {
+    T o1[KT], o2[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const T U = ok[k] ? tile[si[k] * LDT + sl[k]] : T(0);
+      const T V = ok[k] ? tile[(H + si[k]) * LDT + sl[k]] : T(0);
+      o1[k] = U + V;
+      o2[k] = U - V;
+    }
+    if (ch.post) {
+      T e1[KT], e2[KT];
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        e1[k] = ok[k] ? ch.post[gidx(si[k], sl[k])] : T(0);
+        e2[k] = ok[k] ? ch.post[gidx(MP - 1 - si[k], sl[k])] : T(0);
+      }
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        o1[k] *= e1[k];
+        o2[k] *= e2[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (ok[k]) {
+        ch.out[gidx(si[k], sl[k])] = o1[k];
+        ch.out[gidx(MP - 1 - si[k], sl[k])] = o2[k];
+      }
   }
-#pragma unroll
-  for (int k = 0; k < KT; ++k)
-    if (ok[k]) ch.out[gi[k]] = v[k];
 }
 
-template <typename T, int AXIS, int MP>
+template <typename T, int AXIS, int MP, bool CEN = false>
 __global__ __launch_bounds__(256) void axis_pass_mfma_kernel(PassParams<T> p) {
   __shared__ T tile[MP * MfmaShape<MP>::LDT];
-  mfma_pass_tile<T, AXIS, MP>(p.c[blockIdx.y], p.mat, p.mat_lo, p.X, p.Y, p.Z, blockIdx.x, tile);
+  mfma_pass_tile<T, AXIS, MP, CEN>(p.c[blockIdx.y], p.mat, p.mat_lo, p.X, p.Y, p.Z, blockIdx.x,
+                                   tile);
 }
 
 // Padded length class for an axis of length M (0 if unsupported).
@@ -419,67 +510,106 @@ PassImpl pass_impl();
 // stores the plane back.  With the x pass that follows, a contour step takes
 // two launches instead of three.  Requires Y and Z in the same length class
 // MP <= 64.
+//
+// CEN (centrosymmetric operators, D[M-1-j][M-1-k] = D[j][k]: PmmmDCT and P1
+// with M = MP a multiple of 32): each line x is split into u = x_j + x_{M-1-j}
+// and v = x_j - x_{M-1-j} (j < M/2); then U = Ps u and V = Pa v with the
+// half-size blocks Ps = (D[j][k] + D[j][M-1-k])/2, Pa = (D[j][k] -
+// D[j][M-1-k])/2, and out_j = U_j + V_j, out_{M-1-j} = U_j - V_j.  This halves
+// the products.  The half-size operator is stored as At_c[k][j] with
+// k < M/2, j < M: columns j < M/2 hold Ps^T, columns M/2 + j hold Pa^T.
+//
+// Every 16-row output tile sums its k groups (4 rows each) from the group
+// farthest from the tile's diagonal to the nearest, so the large near-diagonal
+// terms are added last and the running sum is rounded fewer times at full
+// magnitude (this matters for FP32 accumulation).
 // ---------------------------------------------------------------------------
-template <int MP>
+template <int MP, bool CEN>
 struct SlabShape {
   static constexpr int NT = 512;
-  static constexpr int P = MP + 4;                 // LDS row pitch (elements)
-  static constexpr int JT = MP / 16;               // 16-row output tiles
-  static constexpr int CT = MP / 16;               // 16-column output tiles
-  static constexpr int KS = MP / 4;                // MFMA k steps
-  static constexpr int NTILE = JT * CT;            // output tiles per GEMM
-  static constexpr int TPW = (NTILE + 7) / 8;      // tiles per wave
+  static constexpr int P = MP + 16;              // LDS pitch: conflict-free B reads
+  static constexpr int JT = MP / 16;             // 16-row output tiles
+  static constexpr int KS = CEN ? MP / 8 : MP / 4;  // MFMA k steps per output tile
+  static constexpr int NTILE = JT * JT;           // output tiles per product
+  static constexpr int TPW = (NTILE + 7) / 8;     // tiles per wave
+  static_assert(!CEN || MP % 32 == 0, "centrosymmetric split needs M % 32 == 0");
 };
 
-// One GEMM of the slab: out[c][j] = sum_k A[j][k] * src[k][c] for the 16 x 16
-// tiles of wave w, with src stored as lds[k * P + c]; results are returned in
-// registers (acc) for the caller to write once every wave has read src.
-template <typename T, int MP>
-__device__ __forceinline__ void slab_gemm(const T* __restrict__ mat,
-                                          const unsigned short* __restrict__ mat_lo, int M,
-                                          const T* lds, int w, int lane,
-                                          typename MfmaOp<T>::v4 (&acc)[SlabShape<MP>::TPW]) {
-  using S = SlabShape<MP>;
+// Operand registers of one wave for one product: A (and its bf16 low part
+// for FP32) of row tile jt in the far-first group order, and the LDS row of
+// each group.
+template <typename T, int MP, bool CEN>
+struct SlabOperands {
+  static constexpr int KS = SlabShape<MP, CEN>::KS;
+  T a[KS];
+  short alo[std::is_same<T, float>::value ? KS : 1];
+  int brow[KS];  // LDS row of the B operand of this lane for each step
+};
+
+template <typename T, int MP, bool CEN>
+__device__ __forceinline__ void slab_load_operands(const T* __restrict__ mat,
+                                                   const unsigned short* __restrict__ lo, int M,
+                                                   int jt, int lane,
+                                                   SlabOperands<T, MP, CEN>& op) {
+  using S = SlabShape<MP, CEN>;
+  constexpr int KS = S::KS, JT = S::JT;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  const int half = JT / 2;
+  // row block of this tile inside its k space (the whole M, or one half)
+  const int jb = CEN ? (jt % half) : jt;
+  const int kbase = CEN ? (jt < half ? 0 : MP / 2) : 0;
+  int ord[KS];
+  far_first_order<KS>(4.0f * jb + 1.5f, ord);
+  const int jrow = jt * 16 + (lane & 15);
+  const int ldm = CEN ? MP : M;  // row length of the stored operator
+#pragma unroll
+  for (int n = 0; n < KS; ++n) {
+    const int i = 4 * ord[n] + (lane >> 4);  // k index inside the k space
+    op.brow[n] = kbase + i;
+    const bool ok = CEN || (i < M && jrow < M);
+    op.a[n] = ok ? mat[i * ldm + jrow] : T(0);
+    if constexpr (SPLIT) op.alo[n] = (ok && lo) ? (short)lo[i * ldm + jrow] : (short)0;
+  }
+}
+
+// One product of the slab for the tiles of wave w: out[j][c] = sum_k A[j][k]
+// src[k][c], src in lds[k * P + c]; the results stay in registers.
+template <typename T, int MP, bool CEN>
+__device__ __forceinline__ void slab_product(const SlabOperands<T, MP, CEN>& op, bool use_lo,
+                                             const T* lds, int w, int lane,
+                                             typename MfmaOp<T>::v4 (&acc)[SlabShape<MP, CEN>::TPW]) {
+  using S = SlabShape<MP, CEN>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int P = S::P, JT = S::JT, KS = S::KS, NTILE = S::NTILE, TPW = S::TPW;
   constexpr bool SPLIT = std::is_same<T, float>::value;
-  const bool use_lo = SPLIT && mat_lo != nullptr;
-  int cur_jt = -1;
-  T a[KS];
-  short alo[SPLIT ? KS : 1];
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
     acc[u] = V4{T(0), T(0), T(0), T(0)};
-    const int q = w + 8 * u;  // tile index: jt = q % JT, ct = q / JT
+    const int q = w + 8 * u;
     if (q >= NTILE) continue;
-    const int jt = q % JT, ct = q / JT;
-    if (jt != cur_jt) {
-      cur_jt = jt;
-      const int jrow = jt * 16 + (lane & 15);
+    const int col = (q / JT) * 16 + (lane & 15);
 #pragma unroll
-      for (int kk = 0; kk < KS; ++kk) {
-        const int i = 4 * kk + (lane >> 4);
-        a[kk] = (i < M && jrow < M) ? mat[i * M + jrow] : T(0);
-        if constexpr (SPLIT)
-          alo[kk] = (use_lo && i < M && jrow < M) ? (short)mat_lo[i * M + jrow] : (short)0;
-      }
-    }
-    const int col = ct * 16 + (lane & 15);
-#pragma unroll
-    for (int c4 = 0; c4 < KS / 4; ++c4) {
+    for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
       T b[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int kk = 4 * c4 + i;
-        b[i] = lds[(4 * kk + (lane >> 4)) * P + col];
-        acc[u] = MfmaOp<T>::mma(a[kk], b[i], acc[u]);
+        const int n = 4 * c4 + i;
+        if (n < KS) {
+          b[i] = lds[op.brow[n] * P + col];
+          acc[u] = MfmaOp<T>::mma(op.a[n], b[i], acc[u]);
+        } else {
+          b[i] = T(0);
+        }
       }
       if constexpr (SPLIT) {
         if (use_lo) {
-          typename MfmaOp<float>::s4 av = {alo[4 * c4], alo[4 * c4 + 1], alo[4 * c4 + 2],
-                                           alo[4 * c4 + 3]};
-          typename MfmaOp<float>::s4 bv = {f32_to_bf16(b[0]), f32_to_bf16(b[1]),
-                                           f32_to_bf16(b[2]), f32_to_bf16(b[3])};
+          typename MfmaOp<float>::s4 av, bv;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int n = 4 * c4 + i;
+            av[i] = n < KS ? op.alo[n] : (short)0;
+            bv[i] = f32_to_bf16(b[i]);
+          }
           acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
         }
       }
@@ -490,112 +620,213 @@ __device__ __forceinline__ void slab_gemm(const T* __restrict__ mat,
 template <typename T>
 struct SlabParams {
   PassChain<T> c[2];
-  const T* mat_z;
+  const T* mat_z;  // dense At (M x M), or At_c (M/2 x M) when centrosymmetric
   const T* mat_y;
   const unsigned short* lo_z;
   const unsigned short* lo_y;
   int X, Y, Z;
 };
 
-template <typename T, int MP>
+template <typename T, int MP, bool CEN>
 __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
-  using S = SlabShape<MP>;
+  using S = SlabShape<MP, CEN>;
   constexpr int NT = S::NT, P = S::P, JT = S::JT, TPW = S::TPW, NTILE = S::NTILE;
-  constexpr int KE = (MP * MP + NT - 1) / NT;  // plane elements per thread
-  __shared__ T lds[MP * P];
+  constexpr int H = MP / 2;
   using V4 = typename MfmaOp<T>::v4;
+  __shared__ T lds[MP * P];
 
   const PassChain<T> ch = p.c[blockIdx.y];
   const int Y = p.Y, Z = p.Z;
   const long base = (long)blockIdx.x * Y * Z;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
 
-  // ---- stage the plane transposed: lds[z * P + y] (zero padding) ----
-  T v[KE];
-  int gy[KE], gz[KE];
-  bool ok[KE];
+  // ---- operands of both products, loaded with the plane ----
+  const int jt = w % JT;  // every tile of this wave shares its row tile
+  SlabOperands<T, MP, CEN> oz, oy;
+  slab_load_operands<T, MP, CEN>(p.mat_z, p.lo_z, Z, jt, lane, oz);
+  slab_load_operands<T, MP, CEN>(p.mat_y, p.lo_y, Y, jt, lane, oy);
+
+  if constexpr (!CEN) {
+    // ---- stage the plane transposed: lds[z * P + y], zero padding ----
+    constexpr int KE = (MP * MP + NT - 1) / NT;
+    T v[KE];
+    int gy[KE], gz[KE];
+    bool ok[KE];
 #pragma unroll
-  for (int k = 0; k < KE; ++k) {
-    const int f = tid + NT * k;
-    const int y = f / MP, z = f - (f / MP) * MP;
-    gy[k] = y;
-    gz[k] = z;
-    ok[k] = f < MP * MP && y < Y && z < Z;
-    v[k] = ok[k] ? ch.in[base + (long)y * Z + z] : T(0);
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      gy[k] = f / MP;
+      gz[k] = f - (f / MP) * MP;
+      ok[k] = f < MP * MP && gy[k] < Y && gz[k] < Z;
+      v[k] = ok[k] ? ch.in[base + (long)gy[k] * Z + gz[k]] : T(0);
+    }
+    if (ch.pre) {
+      T e[KE];
+#pragma unroll
+      for (int k = 0; k < KE; ++k) e[k] = ok[k] ? ch.pre[base + (long)gy[k] * Z + gz[k]] : T(0);
+#pragma unroll
+      for (int k = 0; k < KE; ++k) v[k] *= e[k];
+    }
+#pragma unroll
+    for (int k = 0; k < KE; ++k)
+      if (tid + NT * k < MP * MP) lds[gz[k] * P + gy[k]] = v[k];
+  } else {
+    // ---- stage z pairs: lds[z'][y] = u, lds[H + z'][y] = v (Y = Z = MP) ----
+    constexpr int KE = MP * H / NT;
+    long g1[KE], g2[KE];
+    T x1[KE], x2[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int y = f / H, z = f - (f / H) * H;
+      g1[k] = base + (long)y * MP + z;
+      g2[k] = base + (long)y * MP + (MP - 1 - z);
+      x1[k] = ch.in[g1[k]];
+      x2[k] = ch.in[g2[k]];
+    }
+    if (ch.pre) {
+      T e1[KE], e2[KE];
+#pragma unroll
+      for (int k = 0; k < KE; ++k) {
+        e1[k] = ch.pre[g1[k]];
+        e2[k] = ch.pre[g2[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < KE; ++k) {
+        x1[k] *= e1[k];
+        x2[k] *= e2[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int y = f / H, z = f - (f / H) * H;
+      lds[z * P + y] = x1[k] + x2[k];
+      lds[(H + z) * P + y] = x1[k] - x2[k];
+    }
   }
-  if (ch.pre) {
-    T e[KE];
-#pragma unroll
-    for (int k = 0; k < KE; ++k) e[k] = ok[k] ? ch.pre[base + (long)gy[k] * Z + gz[k]] : T(0);
-#pragma unroll
-    for (int k = 0; k < KE; ++k) v[k] *= e[k];
-  }
-#pragma unroll
-  for (int k = 0; k < KE; ++k)
-    if (tid + NT * k < MP * MP) lds[gz[k] * P + gy[k]] = v[k];
   __syncthreads();
 
-  // ---- z pass: t[y][j] = sum_z Dz[j][z] * in[y][z], src lds[z][y] ----
+  const bool lo_z = SPLIT && p.lo_z != nullptr, lo_y = SPLIT && p.lo_y != nullptr;
+  // ---- z product: rows j (Dz output, or U/V), columns y ----
   V4 acc[TPW];
-  slab_gemm<T, MP>(p.mat_z, p.lo_z, Z, lds, w, lane, acc);
+  slab_product<T, MP, CEN>(oz, lo_z, lds, w, lane, acc);
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
     const int q = w + 8 * u;
     if (q < NTILE) {
-      const int jt = q % JT, ct = q / JT;
-      const int col = ct * 16 + (lane & 15);  // y
+      const int col = (q / JT) * 16 + (lane & 15);  // y
 #pragma unroll
       for (int r = 0; r < 4; ++r) lds[col * P + jt * 16 + MfmaOp<T>::row(lane, r)] = acc[u][r];
     }
   }
   __syncthreads();
 
-  // ---- y pass: o[j][z] = sum_y Dy[j][y] * t[y][z], src lds[y][z] ----
-  slab_gemm<T, MP>(p.mat_y, p.lo_y, Y, lds, w, lane, acc);
+  if constexpr (CEN) {
+    // lds[y][j] holds U (j < H) and V (j >= H) along z; rebuild the z values
+    // and form the y pairs: lds[y'][z] = u, lds[H + y'][z] = v.
+    constexpr int KB = (H * H + NT - 1) / NT;  // 2 x 2 blocks per thread
+    T nv[KB][4];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int f = tid + NT * k;
+      if (f >= H * H) continue;
+      const int y = f / H, z = f - (f / H) * H;
+      const int yb = MP - 1 - y;
+      const T U1 = lds[y * P + z], V1 = lds[y * P + H + z];
+      const T U2 = lds[yb * P + z], V2 = lds[yb * P + H + z];
+      const T t11 = U1 + V1, t12 = U1 - V1;  // (y, z), (y, MP-1-z)
+      const T t21 = U2 + V2, t22 = U2 - V2;  // (yb, z), (yb, MP-1-z)
+      nv[k][0] = t11 + t21;  // u at (y, z)
+      nv[k][1] = t12 + t22;  // u at (y, MP-1-z)
+      nv[k][2] = t11 - t21;  // v at (y, z)
+      nv[k][3] = t12 - t22;  // v at (y, MP-1-z)
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int f = tid + NT * k;
+      if (f >= H * H) continue;
+      const int y = f / H, z = f - (f / H) * H;
+      lds[y * P + z] = nv[k][0];
+      lds[y * P + MP - 1 - z] = nv[k][1];
+      lds[(H + y) * P + z] = nv[k][2];
+      lds[(H + y) * P + MP - 1 - z] = nv[k][3];
+    }
+    __syncthreads();
+  }
+
+  // ---- y product: rows j' (Dy output, or U/V), columns z ----
+  slab_product<T, MP, CEN>(oy, lo_y, lds, w, lane, acc);
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
     const int q = w + 8 * u;
     if (q < NTILE) {
-      const int jt = q % JT, ct = q / JT;
-      const int col = ct * 16 + (lane & 15);  // z
+      const int col = (q / JT) * 16 + (lane & 15);  // z
 #pragma unroll
       for (int r = 0; r < 4; ++r) lds[(jt * 16 + MfmaOp<T>::row(lane, r)) * P + col] = acc[u][r];
     }
   }
   __syncthreads();
 
-  // ---- store the plane (row-major y, z), optional post multiply ----
+  if constexpr (!CEN) {
+    constexpr int KE = (MP * MP + NT - 1) / NT;
 #pragma unroll
-  for (int k = 0; k < KE; ++k) v[k] = lds[gy[k] * P + gz[k]];
-  if (ch.post) {
-    T e[KE];
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int y = f / MP, z = f - (f / MP) * MP;
+      if (f < MP * MP && y < Y && z < Z) {
+        const long g = base + (long)y * Z + z;
+        T v = lds[y * P + z];
+        if (ch.post) v *= ch.post[g];
+        ch.out[g] = v;
+      }
+    }
+  } else {
+    constexpr int KE = MP * H / NT;
 #pragma unroll
-    for (int k = 0; k < KE; ++k) e[k] = ok[k] ? ch.post[base + (long)gy[k] * Z + gz[k]] : T(0);
-#pragma unroll
-    for (int k = 0; k < KE; ++k) v[k] *= e[k];
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int y = f / MP, z = f - (f / MP) * MP;  // y < H
+      const T U = lds[y * P + z], V = lds[(H + y) * P + z];
+      const long g1 = base + (long)y * MP + z, g2 = base + (long)(MP - 1 - y) * MP + z;
+      T o1 = U + V, o2 = U - V;
+      if (ch.post) {
+        o1 *= ch.post[g1];
+        o2 *= ch.post[g2];
+      }
+      ch.out[g1] = o1;
+      ch.out[g2] = o2;
+    }
   }
-#pragma unroll
-  for (int k = 0; k < KE; ++k)
-    if (ok[k]) ch.out[base + (long)gy[k] * Z + gz[k]] = v[k];
 }
 
 bool slab_enabled();
 
 // Launches the fused z + y pass when Y and Z share a class <= 64; returns
 // hipErrorNotSupported otherwise (the caller then runs the two passes).
+// `cen` selects the centrosymmetric kernel; the caller passes the half-size
+// operators in that case and guarantees Y == Z == MP with MP % 32 == 0.
 template <typename T>
-hipError_t launch_slab_zy(const SlabParams<T>& p, int nchains, hipStream_t s) {
+hipError_t launch_slab_zy(const SlabParams<T>& p, int nchains, bool cen, hipStream_t s) {
   const int cy = pass_class(p.Y), cz = pass_class(p.Z);
   if (!slab_enabled() || cy != cz || cy > 64 || cy == 0 || nchains < 1 || nchains > 2)
     return hipErrorNotSupported;
   dim3 grid((unsigned)p.X, (unsigned)nchains, 1);
+  if (cen) {
+    if (p.Y != cy || p.Z != cz || cy % 32 != 0) return hipErrorInvalidValue;
+    if (cy == 32) hipLaunchKernelGGL((slab_zy_kernel<T, 32, true>), grid, dim3(512), 0, s, p);
+    else hipLaunchKernelGGL((slab_zy_kernel<T, 64, true>), grid, dim3(512), 0, s, p);
+    return hipGetLastError();
+  }
   switch (cy) {
-    case 16: hipLaunchKernelGGL((slab_zy_kernel<T, 16>), grid, dim3(512), 0, s, p); break;
-    case 32: hipLaunchKernelGGL((slab_zy_kernel<T, 32>), grid, dim3(512), 0, s, p); break;
-    case 48: hipLaunchKernelGGL((slab_zy_kernel<T, 48>), grid, dim3(512), 0, s, p); break;
-    default: hipLaunchKernelGGL((slab_zy_kernel<T, 64>), grid, dim3(512), 0, s, p); break;
+    case 16: hipLaunchKernelGGL((slab_zy_kernel<T, 16, false>), grid, dim3(512), 0, s, p); break;
+    case 32: hipLaunchKernelGGL((slab_zy_kernel<T, 32, false>), grid, dim3(512), 0, s, p); break;
+    case 48: hipLaunchKernelGGL((slab_zy_kernel<T, 48, false>), grid, dim3(512), 0, s, p); break;
+    default: hipLaunchKernelGGL((slab_zy_kernel<T, 64, false>), grid, dim3(512), 0, s, p); break;
   }
   return hipGetLastError();
 }
@@ -605,14 +836,24 @@ hipError_t launch_axis_pass_mp(const PassParams<T>& p, int nchains, hipStream_t 
   const int M = AXIS == 0 ? p.X : (AXIS == 1 ? p.Y : p.Z);
   const long nlines = (long)p.X * p.Y * p.Z / M;
   if constexpr (MP <= 64) {
-    if (pass_impl() == PassImpl::Mfma) {
+    if (p.cen || pass_impl() == PassImpl::Mfma) {
       using S = MfmaShape<MP>;
       const long nblk = (nlines + S::TL - 1) / S::TL;
       dim3 grid((unsigned)nblk, (unsigned)nchains, 1);
+      if (p.cen) {
+        if constexpr (MP % 32 == 0) {
+          if (M != MP) return hipErrorInvalidValue;
+          hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP, true>), grid, dim3(256), 0, s,
+                             p);
+          return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+      }
       hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP>), grid, dim3(256), 0, s, p);
       return hipGetLastError();
     }
   }
+  if (p.cen) return hipErrorInvalidValue;
   using S = PassShape<MP>;
   const long nblk = (nlines + S::TL - 1) / S::TL;
   dim3 grid((unsigned)nblk, (unsigned)nchains, 1);

@@ -86,6 +86,8 @@ struct OpSet {
   double ds;
   void* dmat[3];     // device, element type of the solver's precision
   void* dmat_lo[3];  // FP32: bf16 bits of D - (float)D (split-precision product)
+  void* dmat_c[3];   // half-size centrosymmetric form (PmmmDCT/P1, M % 32 == 0) or null
+  void* dmat_c_lo[3];
   void* dmat_dL[3];  // dD/dL per axis, built on first use (cell stress)
 };
 
@@ -128,35 +130,59 @@ void free_ops(crys_solver* h) {
     for (int a = 0; a < 3; ++a) {
       if (o.dmat[a]) (void)hipFree(o.dmat[a]);
       if (o.dmat_lo[a]) (void)hipFree(o.dmat_lo[a]);
+      if (o.dmat_c[a]) (void)hipFree(o.dmat_c[a]);
+      if (o.dmat_c_lo[a]) (void)hipFree(o.dmat_c_lo[a]);
       if (o.dmat_dL[a]) (void)hipFree(o.dmat_dL[a]);
     }
   h->ops.clear();
   h->cur = -1;
 }
 
+bool centro_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("CRYS_CENTRO");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
+// Uploads one operator in the solver's precision (and, for FP32, the bf16
+// bits of its low part to *lo when lo is not null).
+int upload_matrix(const crys_solver* h, const std::vector<double>& At, void** dst, void** lo) {
+  const size_t cnt = At.size();
+  HIPCHK(hipMalloc(dst, cnt * elem_size(h->precision)));
+  if (h->precision == CRYS_FP32) {
+    std::vector<float> f(At.begin(), At.end());
+    HIPCHK(hipMemcpy(*dst, f.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+    if (lo) {
+      std::vector<unsigned short> b(cnt);
+      for (size_t i = 0; i < cnt; ++i) b[i] = crys::bf16_bits((float)(At[i] - (double)f[i]));
+      HIPCHK(hipMalloc(lo, cnt * sizeof(unsigned short)));
+      HIPCHK(hipMemcpy(*lo, b.data(), cnt * sizeof(unsigned short), hipMemcpyHostToDevice));
+    }
+  } else {
+    HIPCHK(hipMemcpy(*dst, At.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return CRYS_OK;
+}
+
 // Builds and uploads the three per-axis operators (or their L derivatives).
 // For FP32 solvers `lo` (if not null) receives the bf16 bits of
-// D - (float)D, the low part of the split-precision product.
-int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3] = nullptr) {
+// D - (float)D, the low part of the split-precision product.  When `cen` is
+// not null it receives the half-size centrosymmetric forms (PmmmDCT and P1
+// axes of length M % 32 == 0; null elsewhere).
+int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3] = nullptr,
+                    void* cen[3] = nullptr, void* cen_lo[3] = nullptr) {
   for (int a = 0; a < 3; ++a) {
     std::vector<double> At;
     const int M = (int)h->nphy[a];
     if (h->kind == CRYS_PMMM_DCT) At = crys::pmmm_axis_operator(M, h->cell[a], ds, dL);
     else if (h->kind == CRYS_P1) At = crys::p1_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
     else At = crys::m3_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
-    const size_t cnt = (size_t)M * M;
-    HIPCHK(hipMalloc(&dst[a], cnt * elem_size(h->precision)));
-    if (h->precision == CRYS_FP32) {
-      std::vector<float> f(At.begin(), At.end());
-      HIPCHK(hipMemcpy(dst[a], f.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
-      if (lo) {
-        std::vector<unsigned short> b(cnt);
-        for (size_t i = 0; i < cnt; ++i) b[i] = crys::bf16_bits((float)(At[i] - (double)f[i]));
-        HIPCHK(hipMalloc(&lo[a], cnt * sizeof(unsigned short)));
-        HIPCHK(hipMemcpy(lo[a], b.data(), cnt * sizeof(unsigned short), hipMemcpyHostToDevice));
-      }
-    } else {
-      HIPCHK(hipMemcpy(dst[a], At.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
+    CHK(upload_matrix(h, At, &dst[a], lo ? &lo[a] : nullptr));
+    if (cen && h->kind != CRYS_3M && M % 32 == 0 && M <= 64 && centro_enabled()) {
+      std::vector<double> C = crys::centro_half(At, M);
+      CHK(upload_matrix(h, C, &cen[a], cen_lo ? &cen_lo[a] : nullptr));
     }
   }
   return CRYS_OK;
@@ -164,8 +190,9 @@ int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3
 
 int build_ops(crys_solver* h, double ds, OpSet* out) {
   out->ds = ds;
-  for (int a = 0; a < 3; ++a) out->dmat[a] = out->dmat_lo[a] = out->dmat_dL[a] = nullptr;
-  return upload_axis_ops(h, ds, false, out->dmat, out->dmat_lo);
+  for (int a = 0; a < 3; ++a)
+    out->dmat[a] = out->dmat_lo[a] = out->dmat_c[a] = out->dmat_c_lo[a] = out->dmat_dL[a] = nullptr;
+  return upload_axis_ops(h, ds, false, out->dmat, out->dmat_lo, out->dmat_c, out->dmat_c_lo);
 }
 
 // One contour step E D E (or a bare D with E == nullptr) for 1 or 2 chains:
@@ -178,32 +205,36 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
   p.X = X;
   p.Y = Y;
   p.Z = Z;
-  // z and y fused on x-slabs when the two axes share a length class
+  // z and y fused on x-slabs when the two axes share a length class; the
+  // half-size centrosymmetric operators when both axes have them
   crys::SlabParams<T> sp{};
   sp.X = X;
   sp.Y = Y;
   sp.Z = Z;
+  const bool cen_zy = op.dmat_c[1] && op.dmat_c[2] && Y == Z;
   for (int c = 0; c < nch; ++c) sp.c[c] = {in[c], scr[c], E[c], nullptr};
-  sp.mat_z = (const T*)op.dmat[2];
-  sp.mat_y = (const T*)op.dmat[1];
-  sp.lo_z = (const unsigned short*)op.dmat_lo[2];
-  sp.lo_y = (const unsigned short*)op.dmat_lo[1];
-  hipError_t e = crys::launch_slab_zy<T>(sp, nch, s);
+  sp.mat_z = (const T*)(cen_zy ? op.dmat_c[2] : op.dmat[2]);
+  sp.mat_y = (const T*)(cen_zy ? op.dmat_c[1] : op.dmat[1]);
+  sp.lo_z = (const unsigned short*)(cen_zy ? op.dmat_c_lo[2] : op.dmat_lo[2]);
+  sp.lo_y = (const unsigned short*)(cen_zy ? op.dmat_c_lo[1] : op.dmat_lo[1]);
+  hipError_t e = crys::launch_slab_zy<T>(sp, nch, cen_zy, s);
   if (e == hipErrorNotSupported) {
-    for (int c = 0; c < nch; ++c) p.c[c] = {in[c], scr[c], E[c], nullptr};
-    p.mat = (const T*)op.dmat[2];
-    p.mat_lo = (const unsigned short*)op.dmat_lo[2];
-    HIPCHK(crys::launch_axis_pass<T>(2, p, nch, s));
-    for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], scr[c], nullptr, nullptr};
-    p.mat = (const T*)op.dmat[1];
-    p.mat_lo = (const unsigned short*)op.dmat_lo[1];
-    HIPCHK(crys::launch_axis_pass<T>(1, p, nch, s));
+    for (int ax = 2; ax >= 1; --ax) {
+      for (int c = 0; c < nch; ++c)
+        p.c[c] = ax == 2 ? crys::PassChain<T>{in[c], scr[c], E[c], nullptr}
+                         : crys::PassChain<T>{scr[c], scr[c], nullptr, nullptr};
+      p.cen = op.dmat_c[ax] != nullptr;
+      p.mat = (const T*)(p.cen ? op.dmat_c[ax] : op.dmat[ax]);
+      p.mat_lo = (const unsigned short*)(p.cen ? op.dmat_c_lo[ax] : op.dmat_lo[ax]);
+      HIPCHK(crys::launch_axis_pass<T>(ax, p, nch, s));
+    }
   } else {
     HIPCHK(e);
   }
   for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], out[c], nullptr, E[c]};
-  p.mat = (const T*)op.dmat[0];
-  p.mat_lo = (const unsigned short*)op.dmat_lo[0];
+  p.cen = op.dmat_c[0] != nullptr;
+  p.mat = (const T*)(p.cen ? op.dmat_c[0] : op.dmat[0]);
+  p.mat_lo = (const unsigned short*)(p.cen ? op.dmat_c_lo[0] : op.dmat_lo[0]);
   HIPCHK(crys::launch_axis_pass<T>(0, p, nch, s));
   return CRYS_OK;
 }
@@ -383,6 +414,8 @@ int crys_set_contour_step(crys_solver_t* h, double ds) {
     for (int a = 0; a < 3; ++a) {
       if (o.dmat[a]) (void)hipFree(o.dmat[a]);
       if (o.dmat_lo[a]) (void)hipFree(o.dmat_lo[a]);
+      if (o.dmat_c[a]) (void)hipFree(o.dmat_c[a]);
+      if (o.dmat_c_lo[a]) (void)hipFree(o.dmat_c_lo[a]);
     }
     return r;
   }

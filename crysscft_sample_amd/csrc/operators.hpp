@@ -32,6 +32,12 @@ std::vector<double> pmmm_axis_operator(int M, double L, double ds, bool dL = fal
 std::vector<double> p1_axis_operator(int N, double L, double ds, bool dL = false);
 std::vector<double> m3_axis_operator(int N_logical, double L, double ds, bool dL = false);
 
+// Half-size form of a centrosymmetric operator (D[M-1-j][M-1-k] = D[j][k],
+// true for PmmmDCT and P1): from At (M x M, At[k*M + j] = D[j][k]) returns
+// At_c (M/2 x M) with At_c[k*M + j] = Ps[j][k] for j < M/2 and
+// At_c[k*M + M/2 + j] = Pa[j][k], Ps/Pa = (D[j][k] +/- D[j][M-1-k]) / 2.
+std::vector<double> centro_half(const std::vector<double>& At, int M);
+
 // bf16 bits of x, round to nearest even (finite x).
 unsigned short bf16_bits(float x);
 
