@@ -8,10 +8,12 @@ Workload: f_A = 0.30, chiN = 25, cubic L = 4.2 Rg, 128^3 logical grid
 (64^3 reduced), ds = 1/200 (NsA = 60, NsB = 140), BCC-seeded Pmmm fields,
 simple mixing 0.1.
 
-Prints ONE JSON line.  The roofline block is for the contour step (the three
-axis passes of both chains), timed with HIP events on the solver's stream;
-the CPU baseline is the oracle (a C restatement of the reference) timed on
-this host on one SCFT iteration of the same workload.
+Prints ONE JSON line.  The roofline block is for the slowest kernel of the
+contour step (the fused z+y slab kernel at this size), both chains per
+launch, its mean launch time measured with HIP events on the solver's stream;
+"contour_step" gives the whole step (slab + x pass) against the step's
+minimum bytes.  The CPU baseline is the oracle (a C restatement of the
+reference) timed on this host on one SCFT iteration of the same workload.
 """
 from __future__ import annotations
 
@@ -29,6 +31,20 @@ sys.path.insert(0, ROOT)
 N_LOG = 128
 FA, XN, L, DS, ACC = 0.30, 25.0, 4.2, 1.0 / 200, 0.1
 HBM_PEAK_GBS = 8000.0
+
+# Kernels of one contour step in issue order, by the number of launches.
+STEP_KERNELS = {
+    2: ["crys::slab_zy_kernel<double,64,true>", "crys::axis_pass_mfma_kernel<double,0,64,true>"],
+    3: ["crys::axis_pass_mfma_kernel<double,2,64>", "crys::axis_pass_mfma_kernel<double,1,64>",
+        "crys::axis_pass_mfma_kernel<double,0,64>"],
+}
+# HBM bytes per dispatch of both chains, from the rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes of this workload (FETCH_SIZE scaled by 1.978 from a copy
+# of known size), profiles/hbm_traffic_pmmm128_fp64_slab.md.
+MEASURED_TRAFFIC = {
+    "crys::slab_zy_kernel<double,64,true>": 12.00e6,
+    "crys::axis_pass_mfma_kernel<double,0,64,true>": 11.86e6,
+}
 
 
 def bytes_per_iteration(n, b, NsA, NsB):
@@ -50,9 +66,9 @@ def run_gpu(precision, steps, warmup, wA, wB):
     tr = run.run(steps, trace=True)
     C.lib().crys_device_synchronize()
     dt = time.perf_counter() - t0
-    ms_step, ms_pass = run.time_step(reps=200)
+    ms_step, ms_launch = run.time_step(reps=200)
     out = dict(iter_s=steps / dt, ms_per_iter=dt / steps * 1e3, trace=tr, warm=warm, NsA=run.NsA,
-               NsB=run.NsB, ms_per_contour_step=ms_step, ms_per_pass=ms_pass)
+               NsB=run.NsB, ms_per_contour_step=ms_step, ms_per_launch=ms_launch)
     run.close()
     solver.close()
     return out
@@ -70,6 +86,28 @@ def cpu_baseline(wA, wB, threads):
     return 1.0 / dt, tr
 
 
+def pmc_workload():
+    """Fixed workload for the rocprofv3 --pmc traffic passes: 2 SCFT iterations
+    of the FP64 workload, then one FP32 diffusion() whose conversion kernel
+    (n doubles read, n floats written) calibrates FETCH_SIZE / WRITE_SIZE."""
+    import crysscft_sample_amd as C
+    from crysscft_sample_amd import fields as F
+
+    wfA, wfB = F.bcc_fields((N_LOG,) * 3, L, FA, XN)
+    wA, wB = F.extract(wfA, "PmmmDCT"), F.extract(wfB, "PmmmDCT")
+    s = C.SolverPmmmDCT((N_LOG,) * 3)
+    run = C.SCFT(s, FA, DS, XN, L, ACC)
+    run.set_fields(wA, wB)
+    run.run(2)
+    s32 = C.SolverPmmmDCT((N_LOG,) * 3, precision="fp32")
+    s32.set_contour_step(DS)
+    s32.upload_iomat(wA)
+    s32.diffusion()
+    s32.synchronize()
+    print("pmc workload done", flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,7 +115,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-propagator leg")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="run the fixed workload of the rocprofv3 --pmc passes and exit")
     args = ap.parse_args()
+    if args.pmc_workload:
+        return pmc_workload()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
               "(the path shards by independent SCFT instances, see DESIGN.md)", file=sys.stderr)
@@ -99,14 +141,25 @@ def main():
 
     NsA, NsB = f64["NsA"], f64["NsB"]
     b_iter64 = bytes_per_iteration(n, 8, NsA, NsB)
-    # contour step of both chains: read q_s, read E, write q_s+1 per chain
-    step_bytes = 2 * 3 * n * 8
-    achieved = step_bytes / (f64["ms_per_contour_step"] * 1e-3) / 1e9
+    # Every kernel of the contour step moves, per chain, one field in, E and
+    # one field out: 3 n b algorithmic bytes, 6 n b per launch of both chains.
+    names = STEP_KERNELS.get(len(f64["ms_per_launch"]), [])
+    k_dom = int(np.argmax(f64["ms_per_launch"]))
+    launch_bytes = 2 * 3 * n * 8
+    t_dom = f64["ms_per_launch"][k_dom]
+    achieved = launch_bytes / (t_dom * 1e-3) / 1e9
+    kernel = names[k_dom] if k_dom < len(names) else f"launch {k_dom}"
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "axis_pass_kernel<double> z+y+x (one contour step, both chains)",
-                "ms_per_launch": round(f64["ms_per_pass"], 6),
-                "bytes_per_launch": step_bytes / 3}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": MEASURED_TRAFFIC.get(kernel),
+                "kernel": kernel + " (both chains)",
+                "ms_per_launch": round(t_dom, 6),
+                "bytes_per_launch": launch_bytes}
+    step_bytes = 2 * 3 * n * 8  # minimum of a contour step: q_s, E, q_s+1 per chain
+    step = {"ms": round(f64["ms_per_contour_step"], 6),
+            "ms_per_launch": {nm: round(t, 6) for nm, t in zip(names, f64["ms_per_launch"])},
+            "algorithmic_bytes": step_bytes,
+            "achieved_gbs": round(step_bytes / (f64["ms_per_contour_step"] * 1e-3) / 1e9, 2)}
 
     cpu = None
     if not args.no_cpu:
@@ -139,6 +192,7 @@ def main():
         "hbm_gbs": round(b_iter64 * f64["iter_s"] / 1e9, 2),
         "bytes_per_iter": b_iter64,
         "roofline": roofline,
+        "contour_step": step,
         "cpu_baseline": cpu,
     }
     if mixed is not None:
@@ -148,7 +202,9 @@ def main():
                          "speedup_vs_fp64": round(mixed["iter_s"] / f64["iter_s"], 3),
                          "hbm_gbs": round(b_iter32 * mixed["iter_s"] / 1e9, 2),
                          "max_rel_dF_vs_fp64_gpu": dF,
-                         "ms_per_contour_step": round(mixed["ms_per_contour_step"], 6)}
+                         "ms_per_contour_step": round(mixed["ms_per_contour_step"], 6),
+                         "ms_per_launch": {nm: round(t, 6) for nm, t in
+                                           zip(names, mixed["ms_per_launch"])}}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -47,6 +47,7 @@ class CrysError(RuntimeError):
 _c_dp = ctypes.POINTER(ctypes.c_double)
 _c_sz = ctypes.c_size_t
 _c_vp = ctypes.c_void_p
+_c_ip = ctypes.POINTER(ctypes.c_int)
 
 # name -> (restype, argtypes); the full set of entry points of crys_solver.h
 SIGNATURES = {
@@ -89,7 +90,7 @@ SIGNATURES = {
         _c_dp, _c_dp, _c_dp]),
     "crys_axis_operator": (ctypes.c_int, [ctypes.c_int, _c_sz, ctypes.c_double, ctypes.c_double,
                                           _c_dp]),
-    "crys_scft_time_step": (ctypes.c_int, [_c_vp, ctypes.c_int, _c_dp, _c_dp]),
+    "crys_scft_time_step": (ctypes.c_int, [_c_vp, ctypes.c_int, _c_dp, _c_dp, _c_ip]),
     "crys_output_fields": (ctypes.c_int, [_c_dp, _c_dp, _c_dp, _c_dp, _c_sz, ctypes.c_char_p,
                                           ctypes.c_int]),
     "crys_input_fields": (ctypes.c_int, [_c_dp, _c_dp, _c_dp, _c_dp, _c_sz, ctypes.c_char_p]),

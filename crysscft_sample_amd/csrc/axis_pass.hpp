@@ -227,6 +227,17 @@ struct MfmaOp<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// Writes one accumulator element to LDS.  FP32 writes go through a volatile
+// pointer so that they are never merged into ds_write2_b32 pairs: hipcc
+// (ROCm 7.2, gfx950) rebases such a pair when its offsets exceed the 8-bit
+// dword range and has emitted a wrong offset for one element of the pair
+// (mfma_pass_tile<float, *, 32, true>: byte 240 written instead of 960).
+template <typename T>
+__device__ __forceinline__ void lds_put(T* p, int idx, T v) {
+  if constexpr (std::is_same<T, float>::value) ((volatile T*)p)[idx] = v;
+  else p[idx] = v;
+}
+
 // float -> bf16 bits, round to nearest even (finite inputs).
 __device__ inline short f32_to_bf16(float x) {
   unsigned u = __float_as_uint(x);
@@ -430,7 +441,8 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
     for (int u = 0; u < TPW; ++u) {
       const int col = (sub * TPW + u) * 16 + (lane & 15);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) tile[(jt * 16 + MfmaOp<T>::row(lane, r)) * LDT + col] = acc[u][r];
+      for (int r = 0; r < 4; ++r)
+        lds_put<T>(tile, (jt * 16 + MfmaOp<T>::row(lane, r)) * LDT + col, acc[u][r]);
     }
   }
   __syncthreads();
@@ -767,7 +779,8 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
     if (q < NTILE) {
       const int col = (q / JT) * 16 + (lane & 15);  // z
 #pragma unroll
-      for (int r = 0; r < 4; ++r) lds[(jt * 16 + MfmaOp<T>::row(lane, r)) * P + col] = acc[u][r];
+      for (int r = 0; r < 4; ++r)
+        lds_put<T>(lds, (jt * 16 + MfmaOp<T>::row(lane, r)) * P + col, acc[u][r]);
     }
   }
   __syncthreads();
@@ -805,6 +818,13 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
 }
 
 bool slab_enabled();
+
+// Whether a contour step fuses its z and y passes on x-slabs (Y and Z share a
+// length class <= 64).
+inline bool slab_applies(int Y, int Z) {
+  const int cy = pass_class(Y), cz = pass_class(Z);
+  return slab_enabled() && cy == cz && cy != 0 && cy <= 64;
+}
 
 // Launches the fused z + y pass when Y and Z share a class <= 64; returns
 // hipErrorNotSupported otherwise (the caller then runs the two passes).

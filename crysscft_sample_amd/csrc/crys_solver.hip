@@ -197,9 +197,13 @@ int build_ops(crys_solver* h, double ds, OpSet* out) {
 
 // One contour step E D E (or a bare D with E == nullptr) for 1 or 2 chains:
 // z pass in -> scr (pre E), y pass in place on scr, x pass scr -> out (post E).
+//
+// `only` >= 0 issues only that launch of the step (0-based, in issue order);
+// *nlaunch (if not null) receives the number of launches of a whole step.
 template <typename T>
 int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const in[2],
-                T* const out[2], T* const scr[2], const T* const E[2], hipStream_t s) {
+                T* const out[2], T* const scr[2], const T* const E[2], hipStream_t s,
+                int only = -1, int* nlaunch = nullptr) {
   const int X = (int)h->nphy[0], Y = (int)h->nphy[1], Z = (int)h->nphy[2];
   crys::PassParams<T> p{};
   p.X = X;
@@ -217,9 +221,13 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
   sp.mat_y = (const T*)(cen_zy ? op.dmat_c[1] : op.dmat[1]);
   sp.lo_z = (const unsigned short*)(cen_zy ? op.dmat_c_lo[2] : op.dmat_lo[2]);
   sp.lo_y = (const unsigned short*)(cen_zy ? op.dmat_c_lo[1] : op.dmat_lo[1]);
-  hipError_t e = crys::launch_slab_zy<T>(sp, nch, cen_zy, s);
-  if (e == hipErrorNotSupported) {
-    for (int ax = 2; ax >= 1; --ax) {
+  int k = 0;
+  if (crys::slab_applies(Y, Z)) {
+    if (only < 0 || only == k) HIPCHK(crys::launch_slab_zy<T>(sp, nch, cen_zy, s));
+    ++k;
+  } else {
+    for (int ax = 2; ax >= 1; --ax, ++k) {
+      if (only >= 0 && only != k) continue;
       for (int c = 0; c < nch; ++c)
         p.c[c] = ax == 2 ? crys::PassChain<T>{in[c], scr[c], E[c], nullptr}
                          : crys::PassChain<T>{scr[c], scr[c], nullptr, nullptr};
@@ -228,14 +236,16 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
       p.mat_lo = (const unsigned short*)(p.cen ? op.dmat_c_lo[ax] : op.dmat_lo[ax]);
       HIPCHK(crys::launch_axis_pass<T>(ax, p, nch, s));
     }
-  } else {
-    HIPCHK(e);
   }
-  for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], out[c], nullptr, E[c]};
-  p.cen = op.dmat_c[0] != nullptr;
-  p.mat = (const T*)(p.cen ? op.dmat_c[0] : op.dmat[0]);
-  p.mat_lo = (const unsigned short*)(p.cen ? op.dmat_c_lo[0] : op.dmat_lo[0]);
-  HIPCHK(crys::launch_axis_pass<T>(0, p, nch, s));
+  if (only < 0 || only == k) {
+    for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], out[c], nullptr, E[c]};
+    p.cen = op.dmat_c[0] != nullptr;
+    p.mat = (const T*)(p.cen ? op.dmat_c[0] : op.dmat[0]);
+    p.mat_lo = (const unsigned short*)(p.cen ? op.dmat_c_lo[0] : op.dmat_lo[0]);
+    HIPCHK(crys::launch_axis_pass<T>(0, p, nch, s));
+  }
+  ++k;
+  if (nlaunch) *nlaunch = k;
   return CRYS_OK;
 }
 
@@ -925,18 +935,18 @@ int crys_axis_operator(int kind, size_t n_logical, double L, double ds, double* 
   return CRYS_OK;
 }
 
-int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* ms_per_pass) {
+int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* ms_per_launch,
+                        int* nkernels) {
   if (!h || reps <= 0) return fail(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
   crys_solver* sv = h->sv;
   if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
   const OpSet& op = sv->ops[sv->cur];
   hipStream_t s = sv->stream;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
   const size_t n = h->n, Ns = h->Ns;
-  int r = CRYS_OK;
-  auto step = [&]() -> int {
+  int nl = 0;
+  // one contour step of both chains (q_0 -> q_1 forward, q+_Ns -> q+_Ns-1
+  // backward), or only its launch `only`
+  auto step = [&](int only) -> int {
     if (sv->precision == CRYS_FP32) {
       float* qf = (float*)h->qf;
       float* qb = (float*)h->qb;
@@ -944,7 +954,7 @@ int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* m
       float* out[2] = {qf + n, qb + (Ns - 1) * n};
       float* sc[2] = {(float*)h->scr[0], (float*)h->scr[1]};
       const float* E[2] = {(const float*)h->EA, (const float*)h->EB};
-      return launch_step<float>(sv, op, 2, in, out, sc, E, s);
+      return launch_step<float>(sv, op, 2, in, out, sc, E, s, only, &nl);
     }
     double* qf = (double*)h->qf;
     double* qb = (double*)h->qb;
@@ -952,19 +962,32 @@ int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* m
     double* out[2] = {qf + n, qb + (Ns - 1) * n};
     double* sc[2] = {(double*)h->scr[0], (double*)h->scr[1]};
     const double* E[2] = {(const double*)h->EA, (const double*)h->EB};
-    return launch_step<double>(sv, op, 2, in, out, sc, E, s);
+    return launch_step<double>(sv, op, 2, in, out, sc, E, s, only, &nl);
   };
-  r = step();  // warm
-  if (r == CRYS_OK) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto timed = [&](int only, double* ms_out) -> int {
+    int r = step(only);  // warm
+    if (r != CRYS_OK) return r;
     (void)hipEventRecord(e0, s);
-    for (int i = 0; i < reps && r == CRYS_OK; ++i) r = step();
+    for (int i = 0; i < reps && r == CRYS_OK; ++i) r = step(only);
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms_per_step) *ms_per_step = ms / reps;
-    if (ms_per_pass) *ms_per_pass = ms / (3.0 * reps);
+    *ms_out = ms / reps;
+    return r;
+  };
+  double whole = 0.0;
+  int r = timed(-1, &whole);
+  if (r == CRYS_OK && ms_per_step) *ms_per_step = whole;
+  for (int k = 0; r == CRYS_OK && k < nl && k < 3; ++k) {
+    double t = 0.0;
+    r = timed(k, &t);
+    if (r == CRYS_OK && ms_per_launch) ms_per_launch[k] = t;
   }
+  if (r == CRYS_OK && nkernels) *nkernels = nl;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return r;

@@ -108,10 +108,14 @@ class SCFT:
         return {"H": H.value, "dH_dwA": gA, "dH_dwB": gB, "dH_dL": dL, "trace_row": row}
 
     def time_step(self, reps: int = 50):
-        """(ms per contour step of both chains, ms per axis-pass launch)."""
-        a, b = ctypes.c_double(), ctypes.c_double()
-        check(lib().crys_scft_time_step(self._h, int(reps), ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        """(ms per contour step of both chains, [ms per launch of each kernel
+        of the step, in issue order]), timed with HIP events on the stream."""
+        a = ctypes.c_double()
+        per = (ctypes.c_double * 3)()
+        nk = ctypes.c_int()
+        check(lib().crys_scft_time_step(self._h, int(reps), ctypes.byref(a), per,
+                                        ctypes.byref(nk)))
+        return a.value, [per[k] for k in range(nk.value)]
 
 
 def diblock_scft_simple_iterate(Nx, Ny, Nz, fA, ds, xN, Lx, Ly, Lz, acceptance, Nstep,

@@ -261,10 +261,16 @@ CRYS_API int crys_axis_operator(int kind, size_t n_logical, double L, double ds,
 
 /* ---- measurement -------------------------------------------------------- */
 
-/* Times `reps` back-to-back launches of one propagation step (both chains,
- * every pass of one contour step) of an SCFT handle on its stream with HIP
- * events; writes the mean milliseconds per step and per pass launch. */
-CRYS_API int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* ms_per_pass);
+/* Times one contour step of both chains of an SCFT handle on its stream with
+ * HIP events: `reps` whole steps back to back, then `reps` launches of each
+ * kernel of the step alone.  Writes the mean milliseconds per step, the mean
+ * milliseconds per launch of each kernel in issue order (ms_per_launch[k],
+ * k < *nkernels; at most 3 entries) and the number of kernels per step (2 when
+ * the z and y passes are fused on x-slabs, else 3).  Any output may be null.
+ * No counterpart in the reference, which times diffusion() with std::clock
+ * (crysSCFT_sample.cpp:389-391). */
+CRYS_API int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step,
+                                 double* ms_per_launch, int* nkernels);
 
 #ifdef __cplusplus
 }
