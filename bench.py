@@ -32,16 +32,20 @@ N_LOG = 128
 FA, XN, L, DS, ACC = 0.30, 25.0, 4.2, 1.0 / 200, 0.1
 HBM_PEAK_GBS = 8000.0
 
-# Kernels of one contour step in issue order, by the number of launches.
+# Kernels of one contour step in issue order, by the number of launches: the
+# one-launch plane phase (all axes in one length class <= 64), the fused z+y
+# slab plus the x pass, or three axis passes.
 STEP_KERNELS = {
+    1: ["crys::plane_phase_kernel<double,64,true,1024>"],
     2: ["crys::slab_zy_kernel<double,64,true>", "crys::axis_pass_mfma_kernel<double,0,64,true>"],
     3: ["crys::axis_pass_mfma_kernel<double,2,64>", "crys::axis_pass_mfma_kernel<double,1,64>",
         "crys::axis_pass_mfma_kernel<double,0,64>"],
 }
 # HBM bytes per dispatch of both chains, from the rocprofv3 FETCH_SIZE and
-# WRITE_SIZE passes of this workload (FETCH_SIZE scaled by 1.978 from a copy
-# of known size), profiles/hbm_traffic_pmmm128_fp64_slab.md.
+# WRITE_SIZE passes of this workload (FETCH_SIZE scaled by 1.975 from a copy
+# of known size), profiles/hbm_traffic_plane_phase_pmmm128_fp64.md.
 MEASURED_TRAFFIC = {
+    "crys::plane_phase_kernel<double,64,true,1024>": 16.17e6,
     "crys::slab_zy_kernel<double,64,true>": 12.00e6,
     "crys::axis_pass_mfma_kernel<double,0,64,true>": 11.86e6,
 }
@@ -141,8 +145,10 @@ def main():
 
     NsA, NsB = f64["NsA"], f64["NsB"]
     b_iter64 = bytes_per_iteration(n, 8, NsA, NsB)
-    # Every kernel of the contour step moves, per chain, one field in, E and
-    # one field out: 3 n b algorithmic bytes, 6 n b per launch of both chains.
+    # The unit of work is the contour step of one chain, 3 n b algorithmic
+    # bytes (q_s, E, q_s+1).  The one-launch phase kernel does one step of
+    # both chains per launch (6 n b); on the two/three-launch paths every
+    # launch also reads a field and E and writes a field per chain (6 n b).
     names = STEP_KERNELS.get(len(f64["ms_per_launch"]), [])
     k_dom = int(np.argmax(f64["ms_per_launch"]))
     launch_bytes = 2 * 3 * n * 8

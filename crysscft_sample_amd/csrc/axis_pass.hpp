@@ -542,9 +542,16 @@ struct SlabShape {
   static constexpr int P = MP + 16;              // LDS pitch: conflict-free B reads
   static constexpr int JT = MP / 16;             // 16-row output tiles
   static constexpr int KS = CEN ? MP / 8 : MP / 4;  // MFMA k steps per output tile
-  static constexpr int NTILE = JT * JT;           // output tiles per product
-  static constexpr int TPW = (NTILE + 7) / 8;     // tiles per wave
+  // Wave w owns row tile w % JT (its operator registers) and the column
+  // tiles s, s + WPJ, ... with s = w / JT < WPJ.
+  static constexpr int WPJ = (NT / 64) / JT;       // waves per row tile
+  static constexpr int TPW = (JT + WPJ - 1) / WPJ;  // column tiles per wave
   static_assert(!CEN || MP % 32 == 0, "centrosymmetric split needs M % 32 == 0");
+  // column tile u of wave w, or -1
+  static __device__ __forceinline__ int col_tile(int w, int u) {
+    const int s = w / JT, ct = s + WPJ * u;
+    return (s < WPJ && ct < JT) ? ct : -1;
+  }
 };
 
 // Operand registers of one wave for one product: A (and its bf16 low part
@@ -592,14 +599,14 @@ __device__ __forceinline__ void slab_product(const SlabOperands<T, MP, CEN>& op,
                                              typename MfmaOp<T>::v4 (&acc)[SlabShape<MP, CEN>::TPW]) {
   using S = SlabShape<MP, CEN>;
   using V4 = typename MfmaOp<T>::v4;
-  constexpr int P = S::P, JT = S::JT, KS = S::KS, NTILE = S::NTILE, TPW = S::TPW;
+  constexpr int P = S::P, KS = S::KS, TPW = S::TPW;
   constexpr bool SPLIT = std::is_same<T, float>::value;
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
     acc[u] = V4{T(0), T(0), T(0), T(0)};
-    const int q = w + 8 * u;
-    if (q >= NTILE) continue;
-    const int col = (q / JT) * 16 + (lane & 15);
+    const int ct = S::col_tile(w, u);
+    if (ct < 0) continue;
+    const int col = ct * 16 + (lane & 15);
 #pragma unroll
     for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
       T b[4];
@@ -642,7 +649,7 @@ struct SlabParams {
 template <typename T, int MP, bool CEN>
 __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
   using S = SlabShape<MP, CEN>;
-  constexpr int NT = S::NT, P = S::P, JT = S::JT, TPW = S::TPW, NTILE = S::NTILE;
+  constexpr int NT = S::NT, P = S::P, JT = S::JT, TPW = S::TPW;
   constexpr int H = MP / 2;
   using V4 = typename MfmaOp<T>::v4;
   __shared__ T lds[MP * P];
@@ -727,9 +734,9 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
-    const int q = w + 8 * u;
-    if (q < NTILE) {
-      const int col = (q / JT) * 16 + (lane & 15);  // y
+    const int ct = S::col_tile(w, u);
+    if (ct >= 0) {
+      const int col = ct * 16 + (lane & 15);  // y
 #pragma unroll
       for (int r = 0; r < 4; ++r) lds[col * P + jt * 16 + MfmaOp<T>::row(lane, r)] = acc[u][r];
     }
@@ -775,9 +782,9 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
-    const int q = w + 8 * u;
-    if (q < NTILE) {
-      const int col = (q / JT) * 16 + (lane & 15);  // z
+    const int ct = S::col_tile(w, u);
+    if (ct >= 0) {
+      const int col = ct * 16 + (lane & 15);  // z
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         lds_put<T>(lds, (jt * 16 + MfmaOp<T>::row(lane, r)) * P + col, acc[u][r]);

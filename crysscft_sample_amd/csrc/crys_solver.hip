@@ -19,6 +19,7 @@
 #include "axis_pass.hpp"
 #include "field_ops.hpp"
 #include "operators.hpp"
+#include "plane_phase.hpp"
 
 namespace {
 
@@ -60,6 +61,16 @@ unsigned grid_for(size_t n) {
 
 namespace crys {
 int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+// CRYS_PHASE=0 records the SCFT iteration with two (or three) launches per
+// contour step instead of the one-launch plane phases (for A/B timing).
+bool phase_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("CRYS_PHASE");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  return on;
+}
 
 // CRYS_SLAB=0 runs the z and y passes as two launches (for A/B timing).
 bool slab_enabled() {
@@ -246,6 +257,59 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
   }
   ++k;
   if (nlaunch) *nlaunch = k;
+  return CRYS_OK;
+}
+
+// Length class of the one-launch phase path for this solver (0: not used).
+int phase_mp(const crys_solver* h) {
+  if (!crys::phase_enabled()) return 0;
+  return crys::phase_class((int)h->nphy[0], (int)h->nphy[1], (int)h->nphy[2]);
+}
+
+// Phase k (0..Ns) of both chains of an SCFT propagation (see plane_phase.hpp):
+// phase 0 starts step 0 from q_0, phase k finishes step k-1 (history entry k)
+// and starts step k, phase Ns only finishes.  Even phases work on x-slabs
+// (row axis y), odd phases on y-slabs (row axis x).  Forward chain: q_f[t]
+// -> q_f[t+1] with E_A for t < NsA; backward chain: q_b[Ns-t] -> q_b[Ns-t-1]
+// with E_B while Ns-t-1 >= NsA.
+template <typename T>
+int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns, size_t NsA,
+                      T* qf, T* qb, T* const scr[2], const T* EA, const T* EB, hipStream_t s) {
+  const int X = (int)h->nphy[0], Y = (int)h->nphy[1], Z = (int)h->nphy[2];
+  const size_t n = (size_t)X * Y * Z;
+  const int mp = phase_mp(h);
+  const bool cen = op.dmat_c[0] && op.dmat_c[1] && op.dmat_c[2] && X == mp && Y == mp && Z == mp;
+  const bool yslab = (k % 2) == 1;
+  crys::PhaseParams<T> p{};
+  const int ra = yslab ? 0 : 1;  // row axis
+  p.mat_r = (const T*)(cen ? op.dmat_c[ra] : op.dmat[ra]);
+  p.mat_c = (const T*)(cen ? op.dmat_c[2] : op.dmat[2]);
+  p.lo_r = (const unsigned short*)(cen ? op.dmat_c_lo[ra] : op.dmat_lo[ra]);
+  p.lo_c = (const unsigned short*)(cen ? op.dmat_c_lo[2] : op.dmat_lo[2]);
+  p.MR = yslab ? X : Y;
+  p.MC = Z;
+  p.plane_stride = yslab ? (long)Z : (long)Y * Z;
+  p.row_stride = yslab ? (long)Y * Z : (long)Z;
+  const int nplanes = yslab ? Y : X;
+  auto Ef = [&](size_t t) { return t < NsA ? EA : EB; };
+  auto Eb = [&](size_t t) { return Ns - t - 1 >= NsA ? EB : EA; };
+  crys::PhaseChain<T>& f = p.c[0];
+  crys::PhaseChain<T>& b = p.c[1];
+  f.in = k == 0 ? qf : scr[0];
+  b.in = k == 0 ? qb + Ns * n : scr[1];
+  if (k >= 1) {
+    f.hist = qf + k * n;
+    f.e_fin = Ef(k - 1);
+    b.hist = qb + (Ns - k) * n;
+    b.e_fin = Eb(k - 1);
+  }
+  if (k < Ns) {
+    f.out = scr[0];
+    f.e_next = Ef(k);
+    b.out = scr[1];
+    b.e_next = Eb(k);
+  }
+  HIPCHK(crys::launch_phase<T>(p, nplanes, 2, mp, cen, s));
   return CRYS_OK;
 }
 
@@ -587,12 +651,17 @@ int scft_record_iteration(crys_scft* h, hipStream_t s, double acc_mix) {
   T* scr[2] = {(T*)h->scr[0], (T*)h->scr[1]};
   const T* EA = (const T*)h->EA;
   const T* EB = (const T*)h->EB;
-  for (size_t t = 0; t < Ns; ++t) {
-    const size_t bo = Ns - t - 1;
-    const T* in[2] = {qf + t * n, qb + (Ns - t) * n};
-    T* out[2] = {qf + (t + 1) * n, qb + bo * n};
-    const T* E[2] = {t < NsA ? EA : EB, bo >= NsA ? EB : EA};
-    CHK(launch_step<T>(sv, op, 2, in, out, scr, E, s));
+  if (phase_mp(sv)) {
+    for (size_t k = 0; k <= Ns; ++k)
+      CHK(launch_phase_step<T>(sv, op, k, Ns, NsA, qf, qb, scr, EA, EB, s));
+  } else {
+    for (size_t t = 0; t < Ns; ++t) {
+      const size_t bo = Ns - t - 1;
+      const T* in[2] = {qf + t * n, qb + (Ns - t) * n};
+      T* out[2] = {qf + (t + 1) * n, qb + bo * n};
+      const T* E[2] = {t < NsA ? EA : EB, bo >= NsA ? EB : EA};
+      CHK(launch_step<T>(sv, op, 2, in, out, scr, E, s));
+    }
   }
   const int nb = crys::red_blocks(n);
   hipLaunchKernelGGL((crys::qsum_partial_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
@@ -944,9 +1013,22 @@ int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* m
   hipStream_t s = sv->stream;
   const size_t n = h->n, Ns = h->Ns;
   int nl = 0;
+  const bool phase = phase_mp(sv) != 0 && Ns >= 2;
   // one contour step of both chains (q_0 -> q_1 forward, q+_Ns -> q+_Ns-1
-  // backward), or only its launch `only`
+  // backward), or only its launch `only`; on the phase path the step is one
+  // full phase (phase 1: finishes step 0 and starts step 1)
   auto step = [&](int only) -> int {
+    if (phase) {
+      nl = 1;
+      if (sv->precision == CRYS_FP32) {
+        float* sc[2] = {(float*)h->scr[0], (float*)h->scr[1]};
+        return launch_phase_step<float>(sv, op, 1, Ns, h->NsA, (float*)h->qf, (float*)h->qb, sc,
+                                        (const float*)h->EA, (const float*)h->EB, s);
+      }
+      double* sc[2] = {(double*)h->scr[0], (double*)h->scr[1]};
+      return launch_phase_step<double>(sv, op, 1, Ns, h->NsA, (double*)h->qf, (double*)h->qb, sc,
+                                       (const double*)h->EA, (const double*)h->EB, s);
+    }
     if (sv->precision == CRYS_FP32) {
       float* qf = (float*)h->qf;
       float* qb = (float*)h->qb;

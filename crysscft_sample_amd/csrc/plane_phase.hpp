@@ -1,0 +1,437 @@
+// One launch per contour step: alternating x-slab / y-slab phases.
+//
+// A contour step is q_{s+1} = E_s Dx Dy Dz E_s q_s.  A workgroup that holds a
+// whole plane of the natural (x, y, z) layout can apply the two axes of that
+// plane: an x-slab (fixed x, rows y, contiguous z) applies Dy and Dz, a y-slab
+// (fixed y, rows x, contiguous z) applies Dx and Dz.  Chaining the steps as
+//
+//   phase 0 (x-slab):  Dz Dy E_0 q_0                    (start step 0)
+//   phase 1 (y-slab):  Dx -> q_1 = E_0 (.) ; Dz Dx E_1 q_1
+//   phase 2 (x-slab):  Dy -> q_2 = E_1 (.) ; Dz Dy E_2 q_2
+//   ...
+//   phase Ns:          finish step Ns-1 -> q_Ns
+//
+// gives every step one launch: each phase first applies its row-axis
+// operator to finish the previous step, multiplies by that step's E, writes
+// the history entry, multiplies by the next step's E and applies the row and
+// column (z) operators of the next step.  The half-finished field lives in one
+// scratch field per chain that each workgroup reads and rewrites in place
+// (a phase touches only its own plane).  Per chain and step this moves 4 n
+// values (scratch in, E, history out, scratch out) instead of the 6 n of the
+// two-launch step.  Every array stays in the natural layout, so the density
+// and stress kernels read the histories unchanged.
+//
+// In LDS the plane is kept row-major (rows = row axis, contiguous columns =
+// z).  Row-axis products read it as the B operand (4 rows x 16 contiguous
+// columns per MFMA step, pitch PB); the z product reads it as the A operand
+// (16 rows x 4 contiguous columns, pitch PA): no access is transposed.
+// With CEN (centrosymmetric operators, all three axes of length MP, a
+// multiple of 32) both products use the half-size u/v split described at
+// SlabShape; the splits and recombinations are fused into the staging, the
+// E/history stage and the final store.
+#pragma once
+
+#include "axis_pass.hpp"
+
+namespace crys {
+
+template <typename T>
+struct PhaseChain {
+  const T* in;      // plane source: q_0 (start phase) or the scratch field
+  T* out;           // scratch field for the next phase, or null (finish only)
+  T* hist;          // history entry written after the finishing product, or null (start)
+  const T* e_fin;   // E of the step being finished (with hist)
+  const T* e_next;  // E of the step being started (with out)
+};
+
+template <typename T>
+struct PhaseParams {
+  PhaseChain<T> c[2];
+  const T* mat_r;  // row-axis operator: dense At (M x M) or At_c (M/2 x M)
+  const T* mat_c;  // z operator
+  const unsigned short* lo_r;  // bf16 low parts (FP32), or null
+  const unsigned short* lo_c;
+  int MR, MC;           // row axis length, z length
+  long plane_stride;    // elements between planes
+  long row_stride;      // elements between rows of a plane
+};
+
+template <typename T, int MP, bool CEN, int NT_ = 512>
+struct PhaseShape {
+  static constexpr int NT = NT_;
+  static constexpr int JT = MP / 16;
+  static constexpr int H = MP / 2;
+  static constexpr int KS = CEN ? MP / 8 : MP / 4;
+  static constexpr int PB = MP + 16;                              // row-product pitch
+  static constexpr int PA = MP + (sizeof(T) == 8 ? 2 : 4);        // z-product pitch
+  // Wave w owns 16-row tile w % JT of the operator and the tiles s, s + WPJ,
+  // ... of the other dimension, s = w / JT < WPJ.
+  static constexpr int WPJ = (NT / 64) / JT >= 1 ? (NT / 64) / JT : 1;
+  static constexpr int TPW = (JT + WPJ - 1) / WPJ;
+  static __device__ __forceinline__ int tile(int w, int u) {
+    const int s = w / JT, t = s + WPJ * u;
+    return (s < WPJ && t < JT) ? t : -1;
+  }
+};
+
+// Row-axis product: out[j][c] = sum_k A[j][k] src[k][c], src at pitch PB.
+// Wave w owns row tile w % JT (operator registers) and the column tiles
+// tile(w, u).  All LDS operands of the wave are read first; the MFMAs of its
+// tiles are then issued interleaved (independent accumulators back to back).
+template <typename T, int MP, bool CEN, int NT>
+__device__ __forceinline__ void phase_row_product(
+    const SlabOperands<T, MP, CEN>& op, bool use_lo, const T* lds, int w, int lane,
+    typename MfmaOp<T>::v4 (&acc)[PhaseShape<T, MP, CEN, NT>::TPW]) {
+  using S = PhaseShape<T, MP, CEN, NT>;
+  using V4 = typename MfmaOp<T>::v4;
+  constexpr int PB = S::PB, KS = S::KS, TPW = S::TPW;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  T b[TPW][KS];
+  bool on[TPW];
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    acc[u] = V4{T(0), T(0), T(0), T(0)};
+    const int ct = S::tile(w, u);
+    on[u] = ct >= 0;
+    const int col = (on[u] ? ct : 0) * 16 + (lane & 15);
+#pragma unroll
+    for (int n = 0; n < KS; ++n) b[u][n] = on[u] ? lds[op.brow[n] * PB + col] : T(0);
+  }
+#pragma unroll
+  for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int n = 4 * c4 + i;
+      if (n < KS) {
+#pragma unroll
+        for (int u = 0; u < TPW; ++u)
+          if (on[u]) acc[u] = MfmaOp<T>::mma(op.a[n], b[u][n], acc[u]);
+      }
+    }
+    if constexpr (SPLIT) {
+      if (use_lo) {
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+          if (!on[u]) continue;
+          typename MfmaOp<float>::s4 av, bv;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int n = 4 * c4 + i;
+            av[i] = n < KS ? op.alo[n] : (short)0;
+            bv[i] = n < KS ? f32_to_bf16(b[u][n]) : (short)0;
+          }
+          acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
+        }
+      }
+    }
+  }
+}
+
+// z product: out[r][j] = sum_k src[r][k] Dz[j][k], src at pitch PA (the A
+// operand), the operator in registers as the B operand.  Wave w owns output
+// column tile w % JT (its operator registers) and the row tiles tile(w, u).
+template <typename T, int MP, bool CEN, int NT>
+__device__ __forceinline__ void phase_col_product(
+    const SlabOperands<T, MP, CEN>& op, bool use_lo, const T* lds, int w, int lane,
+    typename MfmaOp<T>::v4 (&acc)[PhaseShape<T, MP, CEN, NT>::TPW]) {
+  using S = PhaseShape<T, MP, CEN, NT>;
+  using V4 = typename MfmaOp<T>::v4;
+  constexpr int PA = S::PA, KS = S::KS, TPW = S::TPW;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  T a[TPW][KS];
+  bool on[TPW];
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    acc[u] = V4{T(0), T(0), T(0), T(0)};
+    const int rt = S::tile(w, u);
+    on[u] = rt >= 0;
+    const int row = (on[u] ? rt : 0) * 16 + (lane & 15);
+#pragma unroll
+    for (int n = 0; n < KS; ++n) a[u][n] = on[u] ? lds[row * PA + op.brow[n]] : T(0);
+  }
+#pragma unroll
+  for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int n = 4 * c4 + i;
+      if (n < KS) {
+#pragma unroll
+        for (int u = 0; u < TPW; ++u)
+          if (on[u]) acc[u] = MfmaOp<T>::mma(a[u][n], op.a[n], acc[u]);
+      }
+    }
+    if constexpr (SPLIT) {
+      if (use_lo) {
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+          if (!on[u]) continue;
+          typename MfmaOp<float>::s4 av, bv;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int n = 4 * c4 + i;
+            av[i] = n < KS ? f32_to_bf16(a[u][n]) : (short)0;
+            bv[i] = n < KS ? op.alo[n] : (short)0;
+          }
+          acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int MP, bool CEN, int NT_ = 512>
+__global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
+  using S = PhaseShape<T, MP, CEN, NT_>;
+  using V4 = typename MfmaOp<T>::v4;
+  constexpr int NT = S::NT, JT = S::JT, H = S::H, PB = S::PB, PA = S::PA, TPW = S::TPW;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  __shared__ T lds[MP * PB];
+
+  const PhaseChain<T> ch = p.c[blockIdx.y];
+  const int MR = p.MR, MC = p.MC;
+  const long RS = p.row_stride;
+  const long base = (long)blockIdx.x * p.plane_stride;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
+  const bool lo_r = SPLIT && p.lo_r != nullptr, lo_c = SPLIT && p.lo_c != nullptr;
+
+  // operators: the row tile w % JT of the row operator, the output column
+  // tile w % JT of the z operator
+  SlabOperands<T, MP, CEN> orow, ocol;
+  slab_load_operands<T, MP, CEN>(p.mat_r, p.lo_r, MR, w % JT, lane, orow);
+  slab_load_operands<T, MP, CEN>(p.mat_c, p.lo_c, MC, w % JT, lane, ocol);
+
+  // ---- every global load up front: the plane and the E values that the
+  // E/history stage needs (same element per task), one memory latency ----
+  // CEN task k: rows r < H and M-1-r of column c; else task k: (r, c)
+  constexpr int TASKS = CEN ? H * MP : MP * MP;
+  constexpr int KE = (TASKS + NT - 1) / NT;
+  constexpr int NV = CEN ? 2 : 1;  // elements per task
+  long g[KE][NV];
+  bool ok[KE];
+  T x[KE][NV], ef[KE][NV], en[KE][NV];
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    const int f = tid + NT * k;
+    const int r = f / MP, c = f - r * MP;
+    ok[k] = f < TASKS && (CEN || (r < MR && c < MC));
+    g[k][0] = base + r * RS + c;
+    if constexpr (CEN) g[k][1] = base + (MP - 1 - r) * RS + c;
+  }
+#pragma unroll
+  for (int k = 0; k < KE; ++k)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) x[k][v] = ok[k] ? ch.in[g[k][v]] : T(0);
+  // E of the finished and of the next step are one array except at the
+  // block junction: then it is read once
+  const bool one_e = fin && next && ch.e_fin == ch.e_next;
+#pragma unroll
+  for (int k = 0; k < KE; ++k)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      ef[k][v] = (fin && ok[k]) ? ch.e_fin[g[k][v]] : T(0);
+      en[k][v] = (next && !one_e && ok[k]) ? ch.e_next[g[k][v]] : ef[k][v];
+    }
+
+  // ---- stage the plane (start: times E of the next step) ----
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    const int f = tid + NT * k;
+    const int r = f / MP, c = f - r * MP;
+    if constexpr (CEN) {
+      // rows r < H: u = x_r + x_{M-1-r}; rows H + r: v = x_r - x_{M-1-r}
+      if (f >= TASKS) continue;
+      const T a = fin ? x[k][0] : x[k][0] * en[k][0];
+      const T b = fin ? x[k][1] : x[k][1] * en[k][1];
+      lds[r * PB + c] = a + b;
+      lds[(H + r) * PB + c] = a - b;
+    } else {
+      if (f < MP * MP) lds[r * PB + c] = fin ? x[k][0] : x[k][0] * en[k][0];
+    }
+  }
+  __syncthreads();
+
+  V4 acc[TPW];
+  const int jt = w % JT;
+  if (fin) {
+    // ---- finish the previous step along the row axis ----
+    phase_row_product<T, MP, CEN, NT>(orow, lo_r, lds, w, lane, acc);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int ct = S::tile(w, u);
+      if (ct >= 0) {
+        const int col = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          lds_put<T>(lds, (jt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[u][r]);
+      }
+    }
+    __syncthreads();
+
+    // ---- q_{s+1} = E_s (.) -> history; times E_{s+1} for the next step ----
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int r = f / MP, c = f - r * MP;
+      if constexpr (CEN) {
+        if (f >= TASKS) continue;
+        const T U = lds[r * PB + c], V = lds[(H + r) * PB + c];
+        T o1 = (U + V) * ef[k][0], o2 = (U - V) * ef[k][1];
+        ch.hist[g[k][0]] = o1;
+        ch.hist[g[k][1]] = o2;
+        if (next) {
+          o1 *= en[k][0];
+          o2 *= en[k][1];
+          lds[r * PB + c] = o1 + o2;
+          lds[(H + r) * PB + c] = o1 - o2;
+        }
+      } else if (f < MP * MP) {
+        T o = T(0);
+        if (ok[k]) {
+          o = lds[r * PB + c] * ef[k][0];
+          ch.hist[g[k][0]] = o;
+          o *= en[k][0];
+        }
+        if (next) lds[r * PB + c] = o;
+      }
+    }
+    if (!next) return;
+    __syncthreads();
+  }
+
+  // ---- start the next step: row-axis product ----
+  phase_row_product<T, MP, CEN, NT>(orow, lo_r, lds, w, lane, acc);
+  __syncthreads();
+  if constexpr (CEN) {
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int ct = S::tile(w, u);
+      if (ct >= 0) {
+        const int col = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          lds_put<T>(lds, (jt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[u][r]);
+      }
+    }
+    __syncthreads();
+    // rows U (j < H) / V (H + j) -> values w at rows j, M-1-j; then the z
+    // pairs of each row: A operand [row][u (c < H) | v (H + c)] at pitch PA
+    constexpr int KB = (H * H + NT - 1) / NT;
+    T nv[KB][4];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int f = tid + NT * k;
+      if (f >= H * H) continue;
+      const int j = f / H, c = f - (f / H) * H;
+      const T P1 = lds[j * PB + c], P2 = lds[(H + j) * PB + c];
+      const T P3 = lds[j * PB + MP - 1 - c], P4 = lds[(H + j) * PB + MP - 1 - c];
+      const T w11 = P1 + P2, w21 = P1 - P2;  // (j, c), (M-1-j, c)
+      const T w12 = P3 + P4, w22 = P3 - P4;  // (j, M-1-c), (M-1-j, M-1-c)
+      nv[k][0] = w11 + w12;  // u (j, c)
+      nv[k][1] = w11 - w12;  // v (j, c)
+      nv[k][2] = w21 + w22;  // u (M-1-j, c)
+      nv[k][3] = w21 - w22;  // v (M-1-j, c)
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int f = tid + NT * k;
+      if (f >= H * H) continue;
+      const int j = f / H, c = f - (f / H) * H;
+      lds[j * PA + c] = nv[k][0];
+      lds[j * PA + H + c] = nv[k][1];
+      lds[(MP - 1 - j) * PA + c] = nv[k][2];
+      lds[(MP - 1 - j) * PA + H + c] = nv[k][3];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int ct = S::tile(w, u);
+      if (ct >= 0) {
+        const int col = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          lds_put<T>(lds, (jt * 16 + MfmaOp<T>::row(lane, r)) * PA + col, acc[u][r]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- z product: rows r, output columns j (U | V for CEN) ----
+  phase_col_product<T, MP, CEN, NT>(ocol, lo_c, lds, w, lane, acc);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    const int rt = S::tile(w, u);
+    if (rt >= 0) {
+      const int col = jt * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        lds_put<T>(lds, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[u][r]);
+    }
+  }
+  __syncthreads();
+
+  // ---- store the scratch plane ----
+  if constexpr (CEN) {
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      if (f >= TASKS) continue;
+      const int r = f / H, j = f - (f / H) * H;
+      const T U = lds[r * PB + j], V = lds[r * PB + H + j];
+      ch.out[base + r * RS + j] = U + V;
+      ch.out[base + r * RS + MP - 1 - j] = U - V;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int r = f / MP, c = f - r * MP;
+      if (f < TASKS && r < MR && c < MC) ch.out[base + r * RS + c] = lds[r * PB + c];
+    }
+  }
+}
+
+// Whether the phase path applies: all three axes in one length class <= 64.
+inline int phase_class(int X, int Y, int Z) {
+  const int c = pass_class(X);
+  if (c == 0 || c > 64 || pass_class(Y) != c || pass_class(Z) != c) return 0;
+  return c;
+}
+
+bool phase_enabled();
+
+// Launches one phase for 1 or 2 chains.  `cen` selects the centrosymmetric
+// kernel (then the caller passes the half-size operators and MR == MC == mp,
+// a multiple of 32).  The caller fills p.c[], the operators and
+// p.MR/MC/strides; `nplanes` is the number of slabs.  Workgroups have 1024
+// threads (16 waves, one 16 x 16 tile each per product at class 64), except
+// the dense class-48/64 kernels, whose operator registers fit 512 threads.
+template <typename T>
+hipError_t launch_phase(const PhaseParams<T>& p, int nplanes, int nchains, int mp, bool cen,
+                        hipStream_t s) {
+  if (nchains < 1 || nchains > 2 || nplanes < 1) return hipErrorInvalidValue;
+  dim3 grid((unsigned)nplanes, (unsigned)nchains, 1);
+  if (cen) {
+    if (p.MR != mp || p.MC != mp) return hipErrorInvalidValue;
+    if (mp == 32)
+      hipLaunchKernelGGL((plane_phase_kernel<T, 32, true, 1024>), grid, dim3(1024), 0, s, p);
+    else if (mp == 64)
+      hipLaunchKernelGGL((plane_phase_kernel<T, 64, true, 1024>), grid, dim3(1024), 0, s, p);
+    else
+      return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
+  switch (mp) {
+    case 16: hipLaunchKernelGGL((plane_phase_kernel<T, 16, false, 1024>), grid, dim3(1024), 0, s, p); break;
+    case 32: hipLaunchKernelGGL((plane_phase_kernel<T, 32, false, 1024>), grid, dim3(1024), 0, s, p); break;
+    case 48: hipLaunchKernelGGL((plane_phase_kernel<T, 48, false, 512>), grid, dim3(512), 0, s, p); break;
+    case 64: hipLaunchKernelGGL((plane_phase_kernel<T, 64, false, 512>), grid, dim3(512), 0, s, p); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace crys

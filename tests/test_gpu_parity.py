@@ -33,7 +33,10 @@ GRIDS = [((8, 12, 16), (4.0, 4.1, 4.3), 0.01),
          ((32, 32, 32), (4.0, 4.0, 4.0), 0.01),
          ((64, 64, 64), (4.2, 4.2, 4.2), 0.005),
          ((48, 24, 32), (4.4, 3.9, 4.0), 0.0075),
-         ((96, 32, 16), (4.6, 4.0, 4.1), 0.01)]
+         ((96, 32, 16), (4.6, 4.0, 4.1), 0.01),
+         # y and z in length class 48 (the fused z+y slab kernel at class 48)
+         ((32, 96, 80), (4.0, 4.5, 4.2), 0.01),
+         ((16, 48, 48), (4.1, 4.3, 4.0), 0.01)]
 
 
 @pytest.mark.parametrize("name", ["PmmmDCT", "3m", "P1"])
@@ -252,3 +255,41 @@ def test_scft_repeated_runs_continue_the_iteration(oracle):
     tr, *_ = ref.scft(c["fA"], c["ds"], c["xN"], c["L"], c["acc"], 10,
                       F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT"))
     assert rel_err(np.concatenate([t1, t2]), tr) < 1e-11
+
+
+# The SCFT iteration records one launch per contour step (alternating x-slab
+# and y-slab phases) when all three reduced axes share a length class <= 64.
+PHASE_CASES = [
+    ("PmmmDCT", (32, 32, 32), (4.0,) * 3, "fp64"),      # class 16
+    ("P1", (48, 48, 48), (4.2,) * 3, "fp64"),           # class 48
+    ("P1", (40, 40, 40), (4.0,) * 3, "fp64"),           # class 48, padded axes
+    ("P1", (24, 32, 32), (4.0, 4.1, 4.3), "fp64"),      # class 32, X != Y, padded x
+    ("P1", (32, 32, 32), (4.1,) * 3, "fp64"),           # class 32, centrosymmetric
+    ("PmmmDCT", (128, 128, 128), (4.2,) * 3, "fp64"),   # class 64, centrosymmetric
+    ("PmmmDCT", (128, 128, 128), (4.2,) * 3, "fp32"),
+    ("P1", (48, 48, 48), (4.2,) * 3, "fp32"),
+]
+
+
+@pytest.mark.parametrize("name,N,cell,prec", PHASE_CASES)
+def test_scft_one_launch_phases(oracle, name, N, cell, prec):
+    fA, xN, ds, acc, nstep = 0.3, 25.0, 0.01, 0.1, 3
+    say("scft phases", name, N, prec)
+    wA, wB = F.bcc_fields(N, cell, fA, xN)
+    a, b = F.extract(wA, name), F.extract(wB, name)
+    ref = oracle.Solver(name, N)
+    tr, _, _, phiA, _ = ref.scft(fA, ds, xN, cell, acc, nstep, a, b)
+    s = C.SOLVERS[name](N, precision=prec)
+    run = C.SCFT(s, fA, ds, xN, cell, acc)
+    run.set_fields(a, b)
+    got = run.run(nstep)
+    gphiA, _, _, _ = run.fields()
+    dF = np.abs(got[:, 0] / tr[:, 0] - 1).max()
+    dM = np.abs(got[:, 5] / tr[:, 5] - 1).max()
+    say(f"  |dF/F| {dF:.2e}  |dMaxInc| {dM:.2e}  |dphiA| {np.abs(gphiA - phiA).max():.2e}")
+    if prec == "fp64":
+        assert dF <= 1e-10 and dM <= 1e-10
+        assert rel_err(gphiA, phiA) < 1e-10
+    else:
+        assert dF < 1e-5
+        assert np.abs(gphiA - phiA).max() < 1e-4
