@@ -36,7 +36,8 @@ HBM_PEAK_GBS = 8000.0
 # one-launch plane phase (all axes in one length class <= 64), the fused z+y
 # slab plus the x pass, or three axis passes.
 STEP_KERNELS = {
-    1: ["crys::plane_phase_kernel<double,64,true,1024>"],
+    1: ["crys::plane_phase_half_kernel<double>" if os.environ.get("CRYS_PHASE_HALF") != "0"
+        else "crys::plane_phase_kernel<double,64,true,1024>"],
     2: ["crys::slab_zy_kernel<double,64,true>", "crys::axis_pass_mfma_kernel<double,0,64,true>"],
     3: ["crys::axis_pass_mfma_kernel<double,2,64>", "crys::axis_pass_mfma_kernel<double,1,64>",
         "crys::axis_pass_mfma_kernel<double,0,64>"],
@@ -45,6 +46,7 @@ STEP_KERNELS = {
 # WRITE_SIZE passes of this workload (FETCH_SIZE scaled by 1.975 from a copy
 # of known size), profiles/hbm_traffic_plane_phase_pmmm128_fp64.md.
 MEASURED_TRAFFIC = {
+    "crys::plane_phase_half_kernel<double>": 16.22e6,
     "crys::plane_phase_kernel<double,64,true,1024>": 16.17e6,
     "crys::slab_zy_kernel<double,64,true>": 12.00e6,
     "crys::axis_pass_mfma_kernel<double,0,64,true>": 11.86e6,

@@ -72,6 +72,16 @@ bool phase_enabled() {
   return on;
 }
 
+// CRYS_PHASE_HALF=0 runs the class-64 centrosymmetric phase on one workgroup
+// per plane instead of two (for A/B timing).
+bool phase_half_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("CRYS_PHASE_HALF");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
 // CRYS_SLAB=0 runs the z and y passes as two launches (for A/B timing).
 bool slab_enabled() {
   static const bool on = [] {
@@ -295,8 +305,10 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
   auto Eb = [&](size_t t) { return Ns - t - 1 >= NsA ? EB : EA; };
   crys::PhaseChain<T>& f = p.c[0];
   crys::PhaseChain<T>& b = p.c[1];
-  f.in = k == 0 ? qf : scr[0];
-  b.in = k == 0 ? qb + Ns * n : scr[1];
+  // phase k writes scratch half k % 2 and reads the half phase k-1 wrote
+  const size_t wr = (k % 2) * n, rd = ((k + 1) % 2) * n;
+  f.in = k == 0 ? qf : scr[0] + rd;
+  b.in = k == 0 ? qb + Ns * n : scr[1] + rd;
   if (k >= 1) {
     f.hist = qf + k * n;
     f.e_fin = Ef(k - 1);
@@ -304,9 +316,9 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
     b.e_fin = Eb(k - 1);
   }
   if (k < Ns) {
-    f.out = scr[0];
+    f.out = scr[0] + wr;
     f.e_next = Ef(k);
-    b.out = scr[1];
+    b.out = scr[1] + wr;
     b.e_next = Eb(k);
   }
   HIPCHK(crys::launch_phase<T>(p, nplanes, 2, mp, cen, s));
@@ -807,8 +819,10 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
   A(&h->EB, n * es);
   A(&h->qf, (h->Ns + 1) * n * es);
   A(&h->qb, (h->Ns + 1) * n * es);
-  A(&h->scr[0], n * es);
-  A(&h->scr[1], n * es);
+  // two fields per chain: the phases alternate between them (a phase may
+  // run on two workgroups per plane, so it cannot rewrite its input in place)
+  A(&h->scr[0], 2 * n * es);
+  A(&h->scr[1], 2 * n * es);
   A((void**)&h->qpart, crys::kMaxRedBlocks * 8);
   A((void**)&h->epart, 4 * crys::kMaxRedBlocks * 8);
   A((void**)&h->scal, 4 * 8);

@@ -14,9 +14,10 @@
 // gives every step one launch: each phase first applies its row-axis
 // operator to finish the previous step, multiplies by that step's E, writes
 // the history entry, multiplies by the next step's E and applies the row and
-// column (z) operators of the next step.  The half-finished field lives in one
-// scratch field per chain that each workgroup reads and rewrites in place
-// (a phase touches only its own plane).  Per chain and step this moves 4 n
+// column (z) operators of the next step.  The half-finished field lives in
+// two scratch fields per chain that the phases alternate between (phase k
+// reads what phase k-1 wrote; the split kernel below has two workgroups per
+// plane, so a phase never rewrites its input in place).  Per chain and step this moves 4 n
 // values (scratch in, E, history out, scratch out) instead of the 6 n of the
 // two-launch step.  Every array stays in the natural layout, so the density
 // and stress kernels read the histories unchanged.
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
   constexpr int NV = CEN ? 2 : 1;  // elements per task
   long g[KE][NV];
   bool ok[KE];
-  T x[KE][NV], ef[KE][NV], en[KE][NV];
+  T x[KE][NV], ef[KE][NV];
 #pragma unroll
   for (int k = 0; k < KE; ++k) {
     const int f = tid + NT * k;
@@ -218,20 +219,18 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
     g[k][0] = base + r * RS + c;
     if constexpr (CEN) g[k][1] = base + (MP - 1 - r) * RS + c;
   }
+  // the plane first, then one E (the finished step's, or the next step's in
+  // a start phase), so that staging waits only for the plane; the other E of
+  // a block junction is read in the E/history stage
+  const T* e_a = fin ? ch.e_fin : ch.e_next;
 #pragma unroll
   for (int k = 0; k < KE; ++k)
 #pragma unroll
     for (int v = 0; v < NV; ++v) x[k][v] = ok[k] ? ch.in[g[k][v]] : T(0);
-  // E of the finished and of the next step are one array except at the
-  // block junction: then it is read once
-  const bool one_e = fin && next && ch.e_fin == ch.e_next;
 #pragma unroll
   for (int k = 0; k < KE; ++k)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      ef[k][v] = (fin && ok[k]) ? ch.e_fin[g[k][v]] : T(0);
-      en[k][v] = (next && !one_e && ok[k]) ? ch.e_next[g[k][v]] : ef[k][v];
-    }
+    for (int v = 0; v < NV; ++v) ef[k][v] = ok[k] ? e_a[g[k][v]] : T(0);
 
   // ---- stage the plane (start: times E of the next step) ----
 #pragma unroll
@@ -241,12 +240,12 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
     if constexpr (CEN) {
       // rows r < H: u = x_r + x_{M-1-r}; rows H + r: v = x_r - x_{M-1-r}
       if (f >= TASKS) continue;
-      const T a = fin ? x[k][0] : x[k][0] * en[k][0];
-      const T b = fin ? x[k][1] : x[k][1] * en[k][1];
+      const T a = fin ? x[k][0] : x[k][0] * ef[k][0];
+      const T b = fin ? x[k][1] : x[k][1] * ef[k][1];
       lds[r * PB + c] = a + b;
       lds[(H + r) * PB + c] = a - b;
     } else {
-      if (f < MP * MP) lds[r * PB + c] = fin ? x[k][0] : x[k][0] * en[k][0];
+      if (f < MP * MP) lds[r * PB + c] = fin ? x[k][0] : x[k][0] * ef[k][0];
     }
   }
   __syncthreads();
@@ -270,6 +269,7 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
     __syncthreads();
 
     // ---- q_{s+1} = E_s (.) -> history; times E_{s+1} for the next step ----
+    const bool junction = next && ch.e_next != ch.e_fin;
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
       const int f = tid + NT * k;
@@ -281,8 +281,8 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
         ch.hist[g[k][0]] = o1;
         ch.hist[g[k][1]] = o2;
         if (next) {
-          o1 *= en[k][0];
-          o2 *= en[k][1];
+          o1 *= junction ? ch.e_next[g[k][0]] : ef[k][0];
+          o2 *= junction ? ch.e_next[g[k][1]] : ef[k][1];
           lds[r * PB + c] = o1 + o2;
           lds[(H + r) * PB + c] = o1 - o2;
         }
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
         if (ok[k]) {
           o = lds[r * PB + c] * ef[k][0];
           ch.hist[g[k][0]] = o;
-          o *= en[k][0];
+          if (next) o *= junction ? ch.e_next[g[k][0]] : ef[k][0];
         }
         if (next) lds[r * PB + c] = o;
       }
@@ -394,6 +394,250 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
   }
 }
 
+// One 16 x 16 output tile with the plane as the B operand: out[j][c] =
+// sum_k A[j][k] src[k][c], src rows k at `pitch`, column col = tile base +
+// lane % 16.  The operator tile is in `op` (rows j of the wave's row tile).
+template <typename T, int MP, bool CEN>
+__device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_b(const SlabOperands<T, MP, CEN>& op,
+                                                              bool use_lo, const T* lds, int pitch,
+                                                              int col) {
+  using V4 = typename MfmaOp<T>::v4;
+  constexpr int KS = SlabShape<MP, CEN>::KS;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  T b[KS];
+#pragma unroll
+  for (int n = 0; n < KS; ++n) b[n] = lds[op.brow[n] * pitch + col];
+  V4 acc = V4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int c4 = 0; c4 < KS / 4; ++c4) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(op.a[4 * c4 + i], b[4 * c4 + i], acc);
+    if constexpr (SPLIT) {
+      if (use_lo) {
+        typename MfmaOp<float>::s4 av, bv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          av[i] = op.alo[4 * c4 + i];
+          bv[i] = f32_to_bf16(b[4 * c4 + i]);
+        }
+        acc = MfmaOp<float>::mma_bf16(av, bv, acc);
+      }
+    }
+  }
+  return acc;
+}
+
+// One 16 x 16 output tile with the plane as the A operand: out[r][j] =
+// sum_k src[r][k] D[j][k], src row r = tile base + lane % 16 at `pitch`.
+template <typename T, int MP, bool CEN>
+__device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a(const SlabOperands<T, MP, CEN>& op,
+                                                              bool use_lo, const T* lds, int pitch,
+                                                              int row) {
+  using V4 = typename MfmaOp<T>::v4;
+  constexpr int KS = SlabShape<MP, CEN>::KS;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  T a[KS];
+#pragma unroll
+  for (int n = 0; n < KS; ++n) a[n] = lds[row * pitch + op.brow[n]];
+  V4 acc = V4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int c4 = 0; c4 < KS / 4; ++c4) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(a[4 * c4 + i], op.a[4 * c4 + i], acc);
+    if constexpr (SPLIT) {
+      if (use_lo) {
+        typename MfmaOp<float>::s4 av, bv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          av[i] = f32_to_bf16(a[4 * c4 + i]);
+          bv[i] = op.alo[4 * c4 + i];
+        }
+        acc = MfmaOp<float>::mma_bf16(av, bv, acc);
+      }
+    }
+  }
+  return acc;
+}
+
+// The class-64 centrosymmetric phase on two workgroups per plane, so that
+// 2 chains x 64 planes fill 256 CUs.  Both workgroups of a plane load it
+// whole and apply the finishing row product whole (the next step's row
+// product needs every row of its result); each then writes the history rows,
+// applies the next row product and the z product, and stores only for its
+// half h of the plane: the rows j and M-1-j with j in [16 h, 16 h + 16), i.e.
+// the U/V row tiles h and 2 + h and the output row tiles h and 3 - h.  Per
+// CU this is 2 product units instead of 3.  The partners of a plane have
+// workgroup ids 8 apart, which places them on one XCD (shared L2).
+// 512 threads: every product after the first is 8 tiles, one per wave.
+template <typename T>
+__global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p) {
+  constexpr int MP = 64, H = 32, NT = 512, JT = 4;
+  using S = PhaseShape<T, MP, true, NT>;
+  using V4 = typename MfmaOp<T>::v4;
+  constexpr int PB = S::PB, PA = S::PA, TPW = S::TPW;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  __shared__ T lds[MP * PB];
+
+  const int id = blockIdx.x;
+  const int h = (id >> 3) & 1;
+  const int plane = ((id >> 4) << 3) | (id & 7);
+  const PhaseChain<T> ch = p.c[blockIdx.y];
+  const long RS = p.row_stride;
+  const long base = (long)plane * p.plane_stride;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
+  const bool lo_r = SPLIT && p.lo_r != nullptr, lo_c = SPLIT && p.lo_c != nullptr;
+
+  // operators: finishing row product (row tile w % 4), next row product
+  // (row tile h or 2 + h), z product (output column tile w / 2)
+  const int r2t = (w & 1) ? 2 + h : h;
+  const int ct2 = w >> 1;
+  const int rt3 = (w & 1) ? 3 - h : h;
+  SlabOperands<T, MP, true> o1, o2, o3;
+  slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, w % JT, lane, o1);
+  slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, r2t, lane, o2);
+  slab_load_operands<T, MP, true>(p.mat_c, p.lo_c, MP, ct2, lane, o3);
+
+  // ---- global loads up front: the plane and its E values ----
+  constexpr int KE = H * MP / NT;
+  long g1[KE], g2[KE];
+  T x1[KE], x2[KE], ef1[KE], ef2[KE];
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    const int f = tid + NT * k;
+    const int r = f / MP, c = f - r * MP;
+    g1[k] = base + r * RS + c;
+    g2[k] = base + (MP - 1 - r) * RS + c;
+  }
+  // the plane first, then one E (the finished step's, or the next step's in
+  // a start phase): unconditional loads in a fixed order, so that staging
+  // waits only for the plane; the other E of a block junction is read later
+  const T* e_a = fin ? ch.e_fin : ch.e_next;
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    x1[k] = ch.in[g1[k]];
+    x2[k] = ch.in[g2[k]];
+  }
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    ef1[k] = e_a[g1[k]];
+    ef2[k] = e_a[g2[k]];
+  }
+
+  // ---- stage u/v rows (start: times E of the next step) ----
+  if (fin) {
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int r = f / MP, c = f - r * MP;
+      lds[r * PB + c] = x1[k] + x2[k];
+      lds[(H + r) * PB + c] = x1[k] - x2[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int r = f / MP, c = f - r * MP;
+      const T a = x1[k] * ef1[k], b = x2[k] * ef2[k];
+      lds[r * PB + c] = a + b;
+      lds[(H + r) * PB + c] = a - b;
+    }
+  }
+  __syncthreads();
+
+  if (fin) {
+    // ---- finishing row product, whole plane ----
+    V4 acc[TPW];
+    phase_row_product<T, MP, true, NT>(o1, lo_r, lds, w, lane, acc);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int ct = S::tile(w, u);
+      if (ct >= 0) {
+        const int col = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          lds_put<T>(lds, ((w % JT) * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[u][r]);
+      }
+    }
+    __syncthreads();
+    // ---- E, history rows of this half, E of the next step ----
+    const bool junction = next && ch.e_next != ch.e_fin;
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int r = f / MP, c = f - r * MP;
+      const T U = lds[r * PB + c], V = lds[(H + r) * PB + c];
+      T o1v = (U + V) * ef1[k], o2v = (U - V) * ef2[k];
+      if ((r >> 4) == h) {
+        ch.hist[g1[k]] = o1v;
+        ch.hist[g2[k]] = o2v;
+      }
+      if (next) {
+        o1v *= junction ? ch.e_next[g1[k]] : ef1[k];
+        o2v *= junction ? ch.e_next[g2[k]] : ef2[k];
+        lds[r * PB + c] = o1v + o2v;
+        lds[(H + r) * PB + c] = o1v - o2v;
+      }
+    }
+    if (!next) return;
+    __syncthreads();
+  }
+
+  // ---- next row product: U/V row tiles h and 2 + h, one tile per wave ----
+  {
+    const V4 acc = mfma_tile_b<T, MP, true>(o2, lo_r, lds, PB, ct2 * 16 + (lane & 15));
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      lds_put<T>(lds, (r2t * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
+  }
+  __syncthreads();
+  // ---- rows j, M-1-j of this half (j = 16 h + f / 32) -> z pairs at PA ----
+  T nv[4];
+  {
+    const int j = 16 * h + tid / H, c = tid - (tid / H) * H;
+    const T P1 = lds[j * PB + c], P2 = lds[(H + j) * PB + c];
+    const T P3 = lds[j * PB + MP - 1 - c], P4 = lds[(H + j) * PB + MP - 1 - c];
+    const T w11 = P1 + P2, w21 = P1 - P2;
+    const T w12 = P3 + P4, w22 = P3 - P4;
+    nv[0] = w11 + w12;
+    nv[1] = w11 - w12;
+    nv[2] = w21 + w22;
+    nv[3] = w21 - w22;
+  }
+  __syncthreads();
+  {
+    const int j = 16 * h + tid / H, c = tid - (tid / H) * H;
+    lds[j * PA + c] = nv[0];
+    lds[j * PA + H + c] = nv[1];
+    lds[(MP - 1 - j) * PA + c] = nv[2];
+    lds[(MP - 1 - j) * PA + H + c] = nv[3];
+  }
+  __syncthreads();
+  // ---- z product: output row tiles h and 3 - h, one tile per wave ----
+  {
+    const V4 acc = mfma_tile_a<T, MP, true>(o3, lo_c, lds, PA, rt3 * 16 + (lane & 15));
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      lds_put<T>(lds, (rt3 * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
+  }
+  __syncthreads();
+  // ---- store this half's rows ----
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int f = tid + NT * k;
+    const int rr = f / H, j = f - (f / H) * H;
+    const int row = rr < 16 ? 16 * h + rr : 48 - 16 * h + (rr - 16);
+    const T U = lds[row * PB + j], V = lds[row * PB + H + j];
+    ch.out[base + row * RS + j] = U + V;
+    ch.out[base + row * RS + MP - 1 - j] = U - V;
+  }
+}
+
+bool phase_half_enabled();
+
 // Whether the phase path applies: all three axes in one length class <= 64.
 inline int phase_class(int X, int Y, int Z) {
   const int c = pass_class(X);
@@ -416,6 +660,13 @@ hipError_t launch_phase(const PhaseParams<T>& p, int nplanes, int nchains, int m
   dim3 grid((unsigned)nplanes, (unsigned)nchains, 1);
   if (cen) {
     if (p.MR != mp || p.MC != mp) return hipErrorInvalidValue;
+    // two workgroups per plane pay off where the products dominate (FP64)
+    if (std::is_same<T, double>::value && mp == 64 && nplanes % 8 == 0 &&
+        phase_half_enabled()) {
+      dim3 g2((unsigned)(2 * nplanes), (unsigned)nchains, 1);
+      hipLaunchKernelGGL((plane_phase_half_kernel<T>), g2, dim3(512), 0, s, p);
+      return hipGetLastError();
+    }
     if (mp == 32)
       hipLaunchKernelGGL((plane_phase_kernel<T, 32, true, 1024>), grid, dim3(1024), 0, s, p);
     else if (mp == 64)
