@@ -118,19 +118,29 @@ __global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
   double s_e1 = 0.0, s_enth = 0.0, s_fk = 0.0, m_inc = 0.0;
   for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < n;
        i += (size_t)gridDim.x * kRedThreads) {
-    double pA = 0.0;
-    for (size_t s = 0; s <= NsA; ++s) {
-      double factor = s == 0 ? 0.5 : (s == NsA ? 0.5 : 1);
-      factor *= ca;
-      pA += factor * (double)qf[s * n + i] * (double)qb[s * n + i];
-    }
-    double pB = 0.0;
-    for (size_t s = 0; s <= NsB; ++s) {
-      double factor = s == 0 ? 0.5 : (s == NsB ? 0.5 : 1);
-      factor *= cb;
-      const size_t k = (NsA + s) * n + i;
-      pB += factor * (double)qf[k] * (double)qb[k];
-    }
+    // sum_s w_s q[s] q+[s] over slots s0..s1 (trapezoid: the end slots
+    // weigh 1/2); four slots per round with independent partial sums so that
+    // 8 loads are in flight per point
+    auto contour_sum = [&](size_t s0, size_t s1) -> double {
+      double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+      size_t s = s0 + 1;
+      for (; s + 4 <= s1; s += 4) {
+        const T a0 = qf[s * n + i], a1 = qf[(s + 1) * n + i];
+        const T a2 = qf[(s + 2) * n + i], a3 = qf[(s + 3) * n + i];
+        const T b0 = qb[s * n + i], b1 = qb[(s + 1) * n + i];
+        const T b2 = qb[(s + 2) * n + i], b3 = qb[(s + 3) * n + i];
+        p0 += (double)a0 * (double)b0;
+        p1 += (double)a1 * (double)b1;
+        p2 += (double)a2 * (double)b2;
+        p3 += (double)a3 * (double)b3;
+      }
+      for (; s < s1; ++s) p0 += (double)qf[s * n + i] * (double)qb[s * n + i];
+      const double ends = 0.5 * ((double)qf[s0 * n + i] * (double)qb[s0 * n + i] +
+                                 (double)qf[s1 * n + i] * (double)qb[s1 * n + i]);
+      return (p0 + p1) + (p2 + p3) + ends;
+    };
+    const double pA = ca * contour_sum(0, NsA);
+    const double pB = cb * contour_sum(NsA, NsA + NsB);
     const double a = wA[i], b = wB[i], x = ksi[i];
     s_e1 += -a * pA - b * pB;
     s_enth += xN * pA * pB;
