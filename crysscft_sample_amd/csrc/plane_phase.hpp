@@ -269,6 +269,7 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
     __syncthreads();
 
     // ---- q_{s+1} = E_s (.) -> history; times E_{s+1} for the next step ----
+    // (non-temporal history stores: read again only by the density kernel)
     const bool junction = next && ch.e_next != ch.e_fin;
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
@@ -278,8 +279,8 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
         if (f >= TASKS) continue;
         const T U = lds[r * PB + c], V = lds[(H + r) * PB + c];
         T o1 = (U + V) * ef[k][0], o2 = (U - V) * ef[k][1];
-        ch.hist[g[k][0]] = o1;
-        ch.hist[g[k][1]] = o2;
+        __builtin_nontemporal_store(o1, &ch.hist[g[k][0]]);
+        __builtin_nontemporal_store(o2, &ch.hist[g[k][1]]);
         if (next) {
           o1 *= junction ? ch.e_next[g[k][0]] : ef[k][0];
           o2 *= junction ? ch.e_next[g[k][1]] : ef[k][1];
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
         T o = T(0);
         if (ok[k]) {
           o = lds[r * PB + c] * ef[k][0];
-          ch.hist[g[k][0]] = o;
+          __builtin_nontemporal_store(o, &ch.hist[g[k][0]]);
           if (next) o *= junction ? ch.e_next[g[k][0]] : ef[k][0];
         }
         if (next) lds[r * PB + c] = o;
@@ -373,7 +374,8 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
   }
   __syncthreads();
 
-  // ---- store the scratch plane ----
+  // ---- store the scratch plane (non-temporal: the next phase reads it from
+  // other XCDs, so it gains nothing from this XCD's L2) ----
   if constexpr (CEN) {
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
@@ -381,15 +383,16 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
       if (f >= TASKS) continue;
       const int r = f / H, j = f - (f / H) * H;
       const T U = lds[r * PB + j], V = lds[r * PB + H + j];
-      ch.out[base + r * RS + j] = U + V;
-      ch.out[base + r * RS + MP - 1 - j] = U - V;
+      __builtin_nontemporal_store(U + V, &ch.out[base + r * RS + j]);
+      __builtin_nontemporal_store(U - V, &ch.out[base + r * RS + MP - 1 - j]);
     }
   } else {
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
       const int f = tid + NT * k;
       const int r = f / MP, c = f - r * MP;
-      if (f < TASKS && r < MR && c < MC) ch.out[base + r * RS + c] = lds[r * PB + c];
+      if (f < TASKS && r < MR && c < MC)
+        __builtin_nontemporal_store(lds[r * PB + c], &ch.out[base + r * RS + c]);
     }
   }
 }
@@ -562,6 +565,8 @@ __global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p)
     }
     __syncthreads();
     // ---- E, history rows of this half, E of the next step ----
+    // (history stores are non-temporal: the entries are read again only by
+    // the density kernel after the whole propagation)
     const bool junction = next && ch.e_next != ch.e_fin;
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
@@ -570,8 +575,8 @@ __global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p)
       const T U = lds[r * PB + c], V = lds[(H + r) * PB + c];
       T o1v = (U + V) * ef1[k], o2v = (U - V) * ef2[k];
       if ((r >> 4) == h) {
-        ch.hist[g1[k]] = o1v;
-        ch.hist[g2[k]] = o2v;
+        __builtin_nontemporal_store(o1v, &ch.hist[g1[k]]);
+        __builtin_nontemporal_store(o2v, &ch.hist[g2[k]]);
       }
       if (next) {
         o1v *= junction ? ch.e_next[g1[k]] : ef1[k];
@@ -624,15 +629,15 @@ __global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p)
       lds_put<T>(lds, (rt3 * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
-  // ---- store this half's rows ----
+  // ---- store this half's rows (non-temporal, as in plane_phase_kernel) ----
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int f = tid + NT * k;
     const int rr = f / H, j = f - (f / H) * H;
     const int row = rr < 16 ? 16 * h + rr : 48 - 16 * h + (rr - 16);
     const T U = lds[row * PB + j], V = lds[row * PB + H + j];
-    ch.out[base + row * RS + j] = U + V;
-    ch.out[base + row * RS + MP - 1 - j] = U - V;
+    __builtin_nontemporal_store(U + V, &ch.out[base + row * RS + j]);
+    __builtin_nontemporal_store(U - V, &ch.out[base + row * RS + MP - 1 - j]);
   }
 }
 
