@@ -462,6 +462,89 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a(const SlabOperands
   return acc;
 }
 
+// Operator registers for half of the k steps of one 16-row tile of a
+// centrosymmetric class-MP operator (far-first order as slab_load_operands):
+// steps 4 kh .. 4 kh + 3 of the MP / 8.
+template <typename T, int MP>
+struct KHalfOperands {
+  T a[4];
+  short alo[std::is_same<T, float>::value ? 4 : 1];
+  int brow[4];
+};
+
+template <typename T, int MP>
+__device__ __forceinline__ void load_operands_khalf(const T* __restrict__ mat,
+                                                    const unsigned short* __restrict__ lo, int jt,
+                                                    int lane, int kh, KHalfOperands<T, MP>& op) {
+  constexpr int KS = MP / 8, JT = MP / 16;
+  static_assert(KS == 8, "k halves of 4 steps");
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  const int half = JT / 2;
+  const int jb = jt % half;
+  const int kbase = jt < half ? 0 : MP / 2;
+  int ord[KS];
+  far_first_order<KS>(4.0f * jb + 1.5f, ord);
+  const int jrow = jt * 16 + (lane & 15);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int o = kh ? ord[4 + i] : ord[i];
+    const int k = 4 * o + (lane >> 4);
+    op.brow[i] = kbase + k;
+    op.a[i] = mat[k * MP + jrow];
+    if constexpr (SPLIT) op.alo[i] = lo ? (short)lo[k * MP + jrow] : (short)0;
+  }
+}
+
+// Partial tile over one k half, the plane as the B operand (see mfma_tile_b).
+template <typename T, int MP>
+__device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_b_khalf(
+    const KHalfOperands<T, MP>& op, bool use_lo, const T* lds, int pitch, int col) {
+  using V4 = typename MfmaOp<T>::v4;
+  T b[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) b[i] = lds[op.brow[i] * pitch + col];
+  V4 acc = V4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(op.a[i], b[i], acc);
+  if constexpr (std::is_same<T, float>::value) {
+    if (use_lo) {
+      typename MfmaOp<float>::s4 av, bv;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        av[i] = op.alo[i];
+        bv[i] = f32_to_bf16(b[i]);
+      }
+      acc = MfmaOp<float>::mma_bf16(av, bv, acc);
+    }
+  }
+  return acc;
+}
+
+// Partial tile over one k half, the plane as the A operand (see mfma_tile_a).
+template <typename T, int MP>
+__device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a_khalf(
+    const KHalfOperands<T, MP>& op, bool use_lo, const T* lds, int pitch, int row) {
+  using V4 = typename MfmaOp<T>::v4;
+  T a[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a[i] = lds[row * pitch + op.brow[i]];
+  V4 acc = V4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(a[i], op.a[i], acc);
+  if constexpr (std::is_same<T, float>::value) {
+    if (use_lo) {
+      typename MfmaOp<float>::s4 av, bv;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        av[i] = f32_to_bf16(a[i]);
+        bv[i] = op.alo[i];
+      }
+      acc = MfmaOp<float>::mma_bf16(av, bv, acc);
+    }
+  }
+  return acc;
+}
+
 // The class-64 centrosymmetric phase on two workgroups per plane, so that
 // 2 chains x 64 planes fill 256 CUs.  Both workgroups of a plane load it
 // whole and apply the finishing row product whole (the next step's row
@@ -471,13 +554,18 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a(const SlabOperands
 // the U/V row tiles h and 2 + h and the output row tiles h and 3 - h.  Per
 // CU this is 2 product units instead of 3.  The partners of a plane have
 // workgroup ids 8 apart, which places them on one XCD (shared L2).
-// 512 threads: every product after the first is 8 tiles, one per wave.
+// 1024 threads (4 waves per SIMD): the finishing product is 16 tiles, one
+// per wave; the other two are 8 tiles each, split over two waves by k half
+// (waves w and w + 8).  The second half's partial tile goes to LDS rows this
+// workgroup does not own (row tile t xor 1, i.e. rows j xor 16) and is added
+// where the tile is read.
 template <typename T>
-__global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p) {
-  constexpr int MP = 64, H = 32, NT = 512, JT = 4;
+__global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p) {
+  constexpr int MP = 64, H = 32, NT = 1024, JT = 4;
   using S = PhaseShape<T, MP, true, NT>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int PB = S::PB, PA = S::PA, TPW = S::TPW;
+  static_assert(TPW == 1, "one finishing tile per wave");
   constexpr bool SPLIT = std::is_same<T, float>::value;
   __shared__ T lds[MP * PB];
 
@@ -491,15 +579,18 @@ __global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p)
   const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
   const bool lo_r = SPLIT && p.lo_r != nullptr, lo_c = SPLIT && p.lo_c != nullptr;
 
-  // operators: finishing row product (row tile w % 4), next row product
-  // (row tile h or 2 + h), z product (output column tile w / 2)
-  const int r2t = (w & 1) ? 2 + h : h;
-  const int ct2 = w >> 1;
-  const int rt3 = (w & 1) ? 3 - h : h;
-  SlabOperands<T, MP, true> o1, o2, o3;
+  // operators: finishing row product (row tile w % 4, all k); next row
+  // product (row tile h or 2 + h, k half kh); z product (output column tile
+  // item / 2, k half kh)
+  const int item = w & 7, kh = w >> 3;
+  const int r2t = (item & 1) ? 2 + h : h;
+  const int ct2 = item >> 1;
+  const int rt3 = (item & 1) ? 3 - h : h;
+  SlabOperands<T, MP, true> o1;
+  KHalfOperands<T, MP> o2, o3;
   slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, w % JT, lane, o1);
-  slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, r2t, lane, o2);
-  slab_load_operands<T, MP, true>(p.mat_c, p.lo_c, MP, ct2, lane, o3);
+  load_operands_khalf<T, MP>(p.mat_r, p.lo_r, r2t, lane, kh, o2);
+  load_operands_khalf<T, MP>(p.mat_c, p.lo_c, ct2, lane, kh, o3);
 
   // ---- global loads up front: the plane and its E values ----
   constexpr int KE = H * MP / NT;
@@ -549,19 +640,16 @@ __global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p)
   __syncthreads();
 
   if (fin) {
-    // ---- finishing row product, whole plane ----
+    // ---- finishing row product, whole plane, one tile per wave ----
     V4 acc[TPW];
     phase_row_product<T, MP, true, NT>(o1, lo_r, lds, w, lane, acc);
     __syncthreads();
+    {
+      const int ct = S::tile(w, 0);
+      const int col = ct * 16 + (lane & 15);
 #pragma unroll
-    for (int u = 0; u < TPW; ++u) {
-      const int ct = S::tile(w, u);
-      if (ct >= 0) {
-        const int col = ct * 16 + (lane & 15);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          lds_put<T>(lds, ((w % JT) * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[u][r]);
-      }
+      for (int r = 0; r < 4; ++r)
+        lds_put<T>(lds, ((w % JT) * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
     }
     __syncthreads();
     // ---- E, history rows of this half, E of the next step ----
@@ -589,21 +677,25 @@ __global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p)
     __syncthreads();
   }
 
-  // ---- next row product: U/V row tiles h and 2 + h, one tile per wave ----
+  // ---- next row product: U/V row tiles h and 2 + h, k halves ----
   {
-    const V4 acc = mfma_tile_b<T, MP, true>(o2, lo_r, lds, PB, ct2 * 16 + (lane & 15));
+    const V4 acc = mfma_tile_b_khalf<T, MP>(o2, lo_r, lds, PB, ct2 * 16 + (lane & 15));
     __syncthreads();
+    const int rt = kh ? (r2t ^ 1) : r2t;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      lds_put<T>(lds, (r2t * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
+      lds_put<T>(lds, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
   // ---- rows j, M-1-j of this half (j = 16 h + f / 32) -> z pairs at PA ----
   T nv[4];
-  {
-    const int j = 16 * h + tid / H, c = tid - (tid / H) * H;
-    const T P1 = lds[j * PB + c], P2 = lds[(H + j) * PB + c];
-    const T P3 = lds[j * PB + MP - 1 - c], P4 = lds[(H + j) * PB + MP - 1 - c];
+  const bool mid = tid < 16 * H;
+  if (mid) {
+    const int j = 16 * h + tid / H, c = tid - (tid / H) * H, jo = j ^ 16;
+    const T P1 = lds[j * PB + c] + lds[jo * PB + c];
+    const T P2 = lds[(H + j) * PB + c] + lds[(H + jo) * PB + c];
+    const T P3 = lds[j * PB + MP - 1 - c] + lds[jo * PB + MP - 1 - c];
+    const T P4 = lds[(H + j) * PB + MP - 1 - c] + lds[(H + jo) * PB + MP - 1 - c];
     const T w11 = P1 + P2, w21 = P1 - P2;
     const T w12 = P3 + P4, w22 = P3 - P4;
     nv[0] = w11 + w12;
@@ -612,7 +704,7 @@ __global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p)
     nv[3] = w21 - w22;
   }
   __syncthreads();
-  {
+  if (mid) {
     const int j = 16 * h + tid / H, c = tid - (tid / H) * H;
     lds[j * PA + c] = nv[0];
     lds[j * PA + H + c] = nv[1];
@@ -620,22 +712,23 @@ __global__ __launch_bounds__(512) void plane_phase_half_kernel(PhaseParams<T> p)
     lds[(MP - 1 - j) * PA + H + c] = nv[3];
   }
   __syncthreads();
-  // ---- z product: output row tiles h and 3 - h, one tile per wave ----
+  // ---- z product: output row tiles h and 3 - h, k halves ----
   {
-    const V4 acc = mfma_tile_a<T, MP, true>(o3, lo_c, lds, PA, rt3 * 16 + (lane & 15));
+    const V4 acc = mfma_tile_a_khalf<T, MP>(o3, lo_c, lds, PA, rt3 * 16 + (lane & 15));
     __syncthreads();
+    const int rt = kh ? (rt3 ^ 1) : rt3;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      lds_put<T>(lds, (rt3 * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
+      lds_put<T>(lds, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
   // ---- store this half's rows (non-temporal, as in plane_phase_kernel) ----
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int f = tid + NT * k;
-    const int rr = f / H, j = f - (f / H) * H;
+  {
+    const int rr = tid / H, j = tid - (tid / H) * H;
     const int row = rr < 16 ? 16 * h + rr : 48 - 16 * h + (rr - 16);
-    const T U = lds[row * PB + j], V = lds[row * PB + H + j];
+    const int ro = row ^ 16;
+    const T U = lds[row * PB + j] + lds[ro * PB + j];
+    const T V = lds[row * PB + H + j] + lds[ro * PB + H + j];
     __builtin_nontemporal_store(U + V, &ch.out[base + row * RS + j]);
     __builtin_nontemporal_store(U - V, &ch.out[base + row * RS + MP - 1 - j]);
   }
@@ -669,7 +762,7 @@ hipError_t launch_phase(const PhaseParams<T>& p, int nplanes, int nchains, int m
     if (std::is_same<T, double>::value && mp == 64 && nplanes % 8 == 0 &&
         phase_half_enabled()) {
       dim3 g2((unsigned)(2 * nplanes), (unsigned)nchains, 1);
-      hipLaunchKernelGGL((plane_phase_half_kernel<T>), g2, dim3(512), 0, s, p);
+      hipLaunchKernelGGL((plane_phase_half_kernel<T>), g2, dim3(1024), 0, s, p);
       return hipGetLastError();
     }
     if (mp == 32)
