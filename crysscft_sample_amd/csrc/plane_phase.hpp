@@ -758,8 +758,8 @@ hipError_t launch_phase(const PhaseParams<T>& p, int nplanes, int nchains, int m
   dim3 grid((unsigned)nplanes, (unsigned)nchains, 1);
   if (cen) {
     if (p.MR != mp || p.MC != mp) return hipErrorInvalidValue;
-    // two workgroups per plane pay off where the products dominate (FP64)
-    if (std::is_same<T, double>::value && mp == 64 && nplanes % 8 == 0 &&
+    // two workgroups per plane: all 256 CUs for 2 chains x 64 planes
+    if (mp == 64 && nplanes % 8 == 0 &&
         phase_half_enabled()) {
       dim3 g2((unsigned)(2 * nplanes), (unsigned)nchains, 1);
       hipLaunchKernelGGL((plane_phase_half_kernel<T>), g2, dim3(1024), 0, s, p);
