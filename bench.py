@@ -211,8 +211,8 @@ def main():
                          "hbm_gbs": round(b_iter32 * mixed["iter_s"] / 1e9, 2),
                          "max_rel_dF_vs_fp64_gpu": dF,
                          "ms_per_contour_step": round(mixed["ms_per_contour_step"], 6),
-                         "ms_per_launch": {nm: round(t, 6) for nm, t in
-                                           zip(names, mixed["ms_per_launch"])}}
+                         "ms_per_launch": {nm.replace("double", "float"): round(t, 6)
+                                           for nm, t in zip(names, mixed["ms_per_launch"])}}
     print(json.dumps(line), flush=True)
     return 0
 

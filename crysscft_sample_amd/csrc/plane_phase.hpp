@@ -567,7 +567,11 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   constexpr int PB = S::PB, PA = S::PA, TPW = S::TPW;
   static_assert(TPW == 1, "one finishing tile per wave");
   constexpr bool SPLIT = std::is_same<T, float>::value;
-  __shared__ T lds[MP * PB];
+  // two plane buffers: every product reads one and writes the other, so
+  // only the barrier that publishes a result separates two stages
+  __shared__ T lds2[2 * MP * PB];
+  T* const buf0 = lds2;
+  T* const buf1 = lds2 + MP * PB;
 
   const int id = blockIdx.x;
   const int h = (id >> 3) & 1;
@@ -624,8 +628,8 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     for (int k = 0; k < KE; ++k) {
       const int f = tid + NT * k;
       const int r = f / MP, c = f - r * MP;
-      lds[r * PB + c] = x1[k] + x2[k];
-      lds[(H + r) * PB + c] = x1[k] - x2[k];
+      buf0[r * PB + c] = x1[k] + x2[k];
+      buf0[(H + r) * PB + c] = x1[k] - x2[k];
     }
   } else {
 #pragma unroll
@@ -633,8 +637,8 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
       const int f = tid + NT * k;
       const int r = f / MP, c = f - r * MP;
       const T a = x1[k] * ef1[k], b = x2[k] * ef2[k];
-      lds[r * PB + c] = a + b;
-      lds[(H + r) * PB + c] = a - b;
+      buf0[r * PB + c] = a + b;
+      buf0[(H + r) * PB + c] = a - b;
     }
   }
   __syncthreads();
@@ -642,14 +646,13 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   if (fin) {
     // ---- finishing row product, whole plane, one tile per wave ----
     V4 acc[TPW];
-    phase_row_product<T, MP, true, NT>(o1, lo_r, lds, w, lane, acc);
-    __syncthreads();
+    phase_row_product<T, MP, true, NT>(o1, lo_r, buf0, w, lane, acc);
     {
       const int ct = S::tile(w, 0);
       const int col = ct * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        lds_put<T>(lds, ((w % JT) * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
+        lds_put<T>(buf1, ((w % JT) * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
     }
     __syncthreads();
     // ---- E, history rows of this half, E of the next step ----
@@ -660,7 +663,7 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     for (int k = 0; k < KE; ++k) {
       const int f = tid + NT * k;
       const int r = f / MP, c = f - r * MP;
-      const T U = lds[r * PB + c], V = lds[(H + r) * PB + c];
+      const T U = buf1[r * PB + c], V = buf1[(H + r) * PB + c];
       T o1v = (U + V) * ef1[k], o2v = (U - V) * ef2[k];
       if ((r >> 4) == h) {
         __builtin_nontemporal_store(o1v, &ch.hist[g1[k]]);
@@ -669,8 +672,8 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
       if (next) {
         o1v *= junction ? ch.e_next[g1[k]] : ef1[k];
         o2v *= junction ? ch.e_next[g2[k]] : ef2[k];
-        lds[r * PB + c] = o1v + o2v;
-        lds[(H + r) * PB + c] = o1v - o2v;
+        buf0[r * PB + c] = o1v + o2v;
+        buf0[(H + r) * PB + c] = o1v - o2v;
       }
     }
     if (!next) return;
@@ -679,47 +682,41 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
 
   // ---- next row product: U/V row tiles h and 2 + h, k halves ----
   {
-    const V4 acc = mfma_tile_b_khalf<T, MP>(o2, lo_r, lds, PB, ct2 * 16 + (lane & 15));
-    __syncthreads();
+    const V4 acc = mfma_tile_b_khalf<T, MP>(o2, lo_r, buf0, PB, ct2 * 16 + (lane & 15));
     const int rt = kh ? (r2t ^ 1) : r2t;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      lds_put<T>(lds, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
+      lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
   // ---- rows j, M-1-j of this half (j = 16 h + f / 32) -> z pairs at PA ----
-  T nv[4];
   const bool mid = tid < 16 * H;
   if (mid) {
     const int j = 16 * h + tid / H, c = tid - (tid / H) * H, jo = j ^ 16;
-    const T P1 = lds[j * PB + c] + lds[jo * PB + c];
-    const T P2 = lds[(H + j) * PB + c] + lds[(H + jo) * PB + c];
-    const T P3 = lds[j * PB + MP - 1 - c] + lds[jo * PB + MP - 1 - c];
-    const T P4 = lds[(H + j) * PB + MP - 1 - c] + lds[(H + jo) * PB + MP - 1 - c];
+    T nv[4];
+    const T P1 = buf1[j * PB + c] + buf1[jo * PB + c];
+    const T P2 = buf1[(H + j) * PB + c] + buf1[(H + jo) * PB + c];
+    const T P3 = buf1[j * PB + MP - 1 - c] + buf1[jo * PB + MP - 1 - c];
+    const T P4 = buf1[(H + j) * PB + MP - 1 - c] + buf1[(H + jo) * PB + MP - 1 - c];
     const T w11 = P1 + P2, w21 = P1 - P2;
     const T w12 = P3 + P4, w22 = P3 - P4;
     nv[0] = w11 + w12;
     nv[1] = w11 - w12;
     nv[2] = w21 + w22;
     nv[3] = w21 - w22;
-  }
-  __syncthreads();
-  if (mid) {
-    const int j = 16 * h + tid / H, c = tid - (tid / H) * H;
-    lds[j * PA + c] = nv[0];
-    lds[j * PA + H + c] = nv[1];
-    lds[(MP - 1 - j) * PA + c] = nv[2];
-    lds[(MP - 1 - j) * PA + H + c] = nv[3];
+    buf0[j * PA + c] = nv[0];
+    buf0[j * PA + H + c] = nv[1];
+    buf0[(MP - 1 - j) * PA + c] = nv[2];
+    buf0[(MP - 1 - j) * PA + H + c] = nv[3];
   }
   __syncthreads();
   // ---- z product: output row tiles h and 3 - h, k halves ----
   {
-    const V4 acc = mfma_tile_a_khalf<T, MP>(o3, lo_c, lds, PA, rt3 * 16 + (lane & 15));
-    __syncthreads();
+    const V4 acc = mfma_tile_a_khalf<T, MP>(o3, lo_c, buf0, PA, rt3 * 16 + (lane & 15));
     const int rt = kh ? (rt3 ^ 1) : rt3;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      lds_put<T>(lds, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
+      lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
   // ---- store this half's rows (non-temporal, as in plane_phase_kernel) ----
@@ -727,8 +724,8 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     const int rr = tid / H, j = tid - (tid / H) * H;
     const int row = rr < 16 ? 16 * h + rr : 48 - 16 * h + (rr - 16);
     const int ro = row ^ 16;
-    const T U = lds[row * PB + j] + lds[ro * PB + j];
-    const T V = lds[row * PB + H + j] + lds[ro * PB + H + j];
+    const T U = buf1[row * PB + j] + buf1[ro * PB + j];
+    const T V = buf1[row * PB + H + j] + buf1[ro * PB + H + j];
     __builtin_nontemporal_store(U + V, &ch.out[base + row * RS + j]);
     __builtin_nontemporal_store(U - V, &ch.out[base + row * RS + MP - 1 - j]);
   }
