@@ -114,6 +114,40 @@ def pmc_workload():
     return 0
 
 
+# The reference's triplet clock (Readme.md:41-76): total std::clock seconds
+# spent in diffusion() over 100 iterations of its 64^3 sample with
+# NsA + NsB = 100, i.e. 2 (NsA + NsB) = 200 triplets per iteration and 20000
+# in all, on one thread of a Xeon E5-2690 v4.
+README_TRIPLET_MS = {"P1": 67.15e3 / 20000, "3m": 11.43e3 / 20000, "PmmmDCT": 17.53e3 / 20000}
+
+
+def triplet_timing(reps):
+    """One JSON line: device milliseconds per diffusion() (one triplet) for
+    the three solvers on the reference's 64^3 sample grid, beside the
+    Readme's per-triplet CPU times."""
+    import crysscft_sample_amd as C
+
+    out = {}
+    rng = np.random.default_rng(5)
+    for name in ("P1", "3m", "PmmmDCT"):
+        for prec in ("fp64", "fp32"):
+            s = C.SOLVERS[name]((64, 64, 64), precision=prec)
+            s.set_cell_para((4.0, 4.0, 4.0))
+            s.set_contour_step(0.01)
+            s.upload_iomat(rng.random(s.get_grid_size_phy()))
+            ms = s.time_diffusion(reps)
+            out[f"{name}_{prec}"] = round(ms, 6)
+            s.close()
+    line = {"metric": "ms per diffusion() triplet, 64^3 logical grid (reference sample)",
+            "unit": "ms", "higher_is_better": False, "reps": reps,
+            "gpu": out,
+            "reference_cpu_readme": {k: round(v, 4) for k, v in README_TRIPLET_MS.items()},
+            "speedup_fp64_vs_readme": {k: round(README_TRIPLET_MS[k] / out[f"{k}_fp64"], 1)
+                                       for k in README_TRIPLET_MS}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,9 +157,13 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-propagator leg")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run the fixed workload of the rocprofv3 --pmc passes and exit")
+    ap.add_argument("--triplet", action="store_true",
+                    help="time single diffusion() triplets (the reference's clock metric) and exit")
     args = ap.parse_args()
     if args.pmc_workload:
         return pmc_workload()
+    if args.triplet:
+        return triplet_timing(200)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
               "(the path shards by independent SCFT instances, see DESIGN.md)", file=sys.stderr)

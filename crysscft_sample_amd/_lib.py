@@ -61,6 +61,7 @@ SIGNATURES = {
     "crys_set_cell_para": (ctypes.c_int, [_c_vp, _c_dp]),
     "crys_set_contour_step": (ctypes.c_int, [_c_vp, ctypes.c_double]),
     "crys_diffusion": (ctypes.c_int, [_c_vp]),
+    "crys_diffusion_time": (ctypes.c_int, [_c_vp, ctypes.c_int, _c_dp]),
     "crys_get_cell_para": (ctypes.c_int, [_c_vp, _c_dp]),
     "crys_get_grid_size_log": (ctypes.c_int, [_c_vp, ctypes.POINTER(_c_sz)]),
     "crys_get_grid_size_phy": (ctypes.c_int, [_c_vp, ctypes.POINTER(_c_sz)]),

@@ -515,6 +515,26 @@ int crys_diffusion(crys_solver_t* h) {
   return diffusion_impl(h);
 }
 
+int crys_diffusion_time(crys_solver_t* h, int reps, double* ms_per_call) {
+  CHK(check_handle(h));
+  if (reps <= 0 || !ms_per_call) return fail(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
+  CHK(diffusion_impl(h));  // warm
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int r = CRYS_OK;
+  (void)hipEventRecord(e0, h->stream);
+  for (int i = 0; i < reps && r == CRYS_OK; ++i) r = diffusion_impl(h);
+  (void)hipEventRecord(e1, h->stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  if (r == CRYS_OK) *ms_per_call = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return r;
+}
+
 int crys_get_cell_para(const crys_solver_t* h, double out[6]) {
   CHK(check_handle(h));
   if (!out) return fail(CRYS_ERR_INVALID_ARGUMENT, "null output");

@@ -80,6 +80,13 @@ class CrysPSSolverBase:
     def diffusion(self):
         check(lib().crys_diffusion(self.handle))
 
+    def time_diffusion(self, reps: int = 200) -> float:
+        """Mean milliseconds per diffusion() call, timed on the device with
+        HIP events over `reps` back-to-back calls (changes the iomat)."""
+        ms = ctypes.c_double()
+        check(lib().crys_diffusion_time(self.handle, int(reps), ctypes.byref(ms)))
+        return ms.value
+
     # -- getters (solver_base.h:22-28) ----------------------------------
     def get_cell_para(self):
         c = np.zeros(6)

@@ -98,6 +98,11 @@ CRYS_API int crys_set_contour_step(crys_solver_t* h, double ds);
  * included, on the device-resident iomat; asynchronous on the handle's
  * stream.  E = exp(-w ds/2) is applied by the caller, as in the reference. */
 CRYS_API int crys_diffusion(crys_solver_t* h);
+/* Times `reps` back-to-back diffusion() calls on the handle's stream with HIP
+ * events and writes the mean milliseconds per call: the device counterpart of
+ * the reference's triplet clock (std::clock around diffusion(),
+ * crysSCFT_sample.cpp:389-391, summed at :512).  Changes the iomat. */
+CRYS_API int crys_diffusion_time(crys_solver_t* h, int reps, double* ms_per_call);
 
 /* Getters (solver_base.h:22-28). */
 CRYS_API int crys_get_cell_para(const crys_solver_t* h, double out[6]);
