@@ -18,6 +18,7 @@
 
 #include "axis_pass.hpp"
 #include "field_ops.hpp"
+#include "m3_glide.hpp"
 #include "operators.hpp"
 #include "plane_phase.hpp"
 
@@ -125,10 +126,19 @@ struct crys_solver {
   void* scratch = nullptr;  // device, n_phy elements of the precision
   std::vector<OpSet> ops;
   int cur = -1;
+  // 3m with non-zero translational parts: diffusion() expands the reduced
+  // field to the logical grid, applies the P1 operators of `full` (a P1
+  // solver on this solver's stream) and extracts the even sub-lattice
+  struct crys_solver* full = nullptr;
+  bool own_stream = true;
+  crys::M3Ops m3{};
 };
 
 struct crys_scft {
   crys_solver* sv = nullptr;
+  // a 3m solver with translational parts: the iteration runs on sv (its P1
+  // solver) with the fields expanded; m3 is the 3m solver the caller passed
+  const crys_solver* m3 = nullptr;
   double fA = 0, fB = 0, ds = 0, xN = 0, acc = 0, L[3] = {0, 0, 0};
   size_t NsA = 0, NsB = 0, Ns = 0, n = 0;
   double *wA = nullptr, *wB = nullptr, *ksi = nullptr, *phiA = nullptr, *phiB = nullptr;
@@ -331,6 +341,20 @@ int check_handle(const crys_solver* h) {
 }
 
 int diffusion_impl(crys_solver* h) {
+  if (h->full) {
+    // 3m with translational parts: expand, P1 on the logical grid, extract
+    if (h->full->cur < 0)
+      return fail(CRYS_ERR_NO_CONTOUR_STEP, "diffusion() before set_contour_step()");
+    const int X = (int)h->nphy[0], Y = (int)h->nphy[1], Z = (int)h->nphy[2];
+    hipLaunchKernelGGL(crys::m3_expand_kernel, dim3(grid_for(h->n_phy)), dim3(256), 0, h->stream,
+                       (const double*)h->iomat, h->full->iomat, h->m3, X, Y, Z);
+    HIPCHK(hipGetLastError());
+    CHK(diffusion_impl(h->full));
+    hipLaunchKernelGGL(crys::m3_extract_kernel, dim3(grid_for(h->n_phy)), dim3(256), 0, h->stream,
+                       (const double*)h->full->iomat, h->iomat, X, Y, Z);
+    HIPCHK(hipGetLastError());
+    return CRYS_OK;
+  }
   if (h->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "diffusion() before set_contour_step()");
   const OpSet& op = h->ops[h->cur];
   const size_t n = h->n_phy;
@@ -427,13 +451,17 @@ int crys_solver_create(int kind, const size_t grid_log[3], const double translat
       return fail(CRYS_ERR_UNSUPPORTED, "physical grid length %zu exceeds 128 on axis %d",
                   grid_log[a], a);
     }
-  if (kind == CRYS_3M)
-    for (int i = 0; i < 9; ++i)
-      if (h->g[i] != 0.0) {
-        delete h;
-        return fail(CRYS_ERR_UNSUPPORTED,
-                    "Solver3m with non-zero translational parts is not implemented on the GPU");
-      }
+  const bool glide = kind == CRYS_3M && crys::m3_nonzero(h->g);
+  if (glide) {
+    const int r = crys::m3_build_ops(h->g, h->nlog, &h->m3);
+    if (r != 0) {
+      delete h;
+      return fail(CRYS_ERR_UNSUPPORTED,
+                  r == 1 ? "Solver3m: a translational part is not a whole number of grid points"
+                         : "Solver3m: the symmetry operations do not map the 2x2y2z sub-lattice "
+                           "onto the grid");
+    }
+  }
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&h->iomat, h->n_phy * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->work, h->n_phy * elem_size(precision));
@@ -444,6 +472,21 @@ int crys_solver_create(int kind, const size_t grid_log[3], const double translat
     return fail(e == hipErrorOutOfMemory ? CRYS_ERR_OUT_OF_MEMORY : CRYS_ERR_HIP,
                 "solver allocation failed: %s", hipGetErrorString(e));
   }
+  if (glide) {
+    crys_solver* p1 = nullptr;
+    const int r = crys_solver_create(CRYS_P1, grid_log, nullptr, precision, 0, &p1);
+    if (r != CRYS_OK) {
+      std::string keep = g_err;
+      crys_solver_destroy(h);
+      g_err = keep;
+      return r;
+    }
+    (void)hipStreamSynchronize(p1->stream);
+    (void)hipStreamDestroy(p1->stream);
+    p1->stream = h->stream;
+    p1->own_stream = false;
+    h->full = p1;
+  }
   *out = h;
   return CRYS_OK;
 }
@@ -451,11 +494,12 @@ int crys_solver_create(int kind, const size_t grid_log[3], const double translat
 int crys_solver_destroy(crys_solver_t* h) {
   if (!h) return CRYS_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->full) crys_solver_destroy(h->full);  // borrows this stream
   free_ops(h);
   if (h->iomat) (void)hipFree(h->iomat);
   if (h->work) (void)hipFree(h->work);
   if (h->scratch) (void)hipFree(h->scratch);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CRYS_OK;
 }
@@ -481,12 +525,18 @@ int crys_set_cell_para(crys_solver_t* h, const double c[6]) {
   }
   free_ops(h);
   h->ds_current = 0.0;
+  if (h->full) CHK(crys_set_cell_para(h->full, h->cell));
   return CRYS_OK;
 }
 
 int crys_set_contour_step(crys_solver_t* h, double ds) {
   CHK(check_handle(h));
   if (!(ds > 0) || !std::isfinite(ds)) return fail(CRYS_ERR_INVALID_ARGUMENT, "ds must be > 0");
+  if (h->full) {  // 3m with translational parts: the P1 solver holds the operators
+    CHK(crys_set_contour_step(h->full, ds));
+    h->ds_current = ds;
+    return CRYS_OK;
+  }
   if (!h->ops.empty() && ds == h->ds_current && h->cur >= 0) return CRYS_OK;
   h->ds_current = ds;
   for (size_t i = 0; i < h->ops.size(); ++i)
@@ -627,10 +677,32 @@ int crys_propagate(crys_solver_t* h, const double* E, const double* init, size_t
   if (!E || !init || !q) return fail(CRYS_ERR_INVALID_ARGUMENT, "null device buffer");
   if (direction != 1 && direction != -1)
     return fail(CRYS_ERR_INVALID_ARGUMENT, "Wrong direction");  // crysSCFT_sample.cpp:381
-  if (h->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
   if (h->precision != CRYS_FP64)
     return fail(CRYS_ERR_UNSUPPORTED, "crys_propagate takes FP64 histories; use an FP64 solver");
   const size_t n = h->n_phy;
+  if (h->full) {
+    // 3m with translational parts: E, diffusion() on the iomat, E per step
+    if (h->full->cur < 0)
+      return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
+    size_t s = direction == 1 ? 0 : Ns;
+    HIPCHK(hipMemcpyAsync(q + s * n, init, n * sizeof(double), hipMemcpyDeviceToDevice,
+                          h->stream));
+    for (size_t it = 0; it < Ns; ++it) {
+      const size_t sn = direction == 1 ? s + 1 : s - 1;
+      hipLaunchKernelGGL(crys::mul_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                         (const double*)(q + s * n), E, h->iomat, n);
+      HIPCHK(hipGetLastError());
+      CHK(diffusion_impl(h));
+      hipLaunchKernelGGL(crys::mul_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                         (const double*)h->iomat, E, q + sn * n, n);
+      HIPCHK(hipGetLastError());
+      s = sn;
+    }
+    HIPCHK(hipMemcpyAsync(h->iomat, q + s * n, n * sizeof(double), hipMemcpyDeviceToDevice,
+                          h->stream));
+    return CRYS_OK;
+  }
+  if (h->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
   const OpSet& op = h->ops[h->cur];
   size_t s = direction == 1 ? 0 : Ns;
   HIPCHK(hipMemcpyAsync(q + s * n, init, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
@@ -813,6 +885,13 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
   crys_scft* h = new (std::nothrow) crys_scft();
   if (!h) return fail(CRYS_ERR_OUT_OF_MEMORY, "host allocation failed");
   h->sv = solver;
+  if (solver->full) {
+    // 3m with translational parts: iterate the expanded fields with the P1
+    // solver; every full-grid mean equals the reduced mean (each reduced point
+    // has 8 images), so the trace is the 3m trace
+    h->sv = solver->full;
+    h->m3 = solver;
+  }
   h->fA = fA;
   h->fB = fB;
   h->ds = ds;
@@ -824,7 +903,7 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
   h->NsA = NsA;
   h->NsB = NsB;
   h->Ns = NsA + NsB;
-  h->n = solver->n_phy;
+  h->n = h->sv->n_phy;
   const size_t n = h->n, es = elem_size(solver->precision);
   hipError_t e = hipSuccess;
   auto A = [&](void** p, size_t bytes) {
@@ -857,7 +936,7 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
   if (solver->precision == CRYS_FP32) scft_fill_ends<float>(h);
   else scft_fill_ends<double>(h);
   e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(solver->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->sv->stream);
   if (e != hipSuccess) {
     scft_free(h);
     delete h;
@@ -878,6 +957,15 @@ int crys_scft_destroy(crys_scft_t* h) {
 int crys_scft_set_fields(crys_scft_t* h, const double* wA, const double* wB) {
   if (!h || !wA || !wB) return fail(CRYS_ERR_INVALID_ARGUMENT, "null argument");
   hipStream_t s = h->sv->stream;
+  std::vector<double> fa, fb;
+  if (h->m3) {
+    fa.resize(h->n);
+    fb.resize(h->n);
+    crys::m3_expand_host(wA, fa.data(), h->m3->m3, h->m3->nphy);
+    crys::m3_expand_host(wB, fb.data(), h->m3->m3, h->m3->nphy);
+    wA = fa.data();
+    wB = fb.data();
+  }
   HIPCHK(hipMemcpyAsync(h->wA, wA, h->n * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(h->wB, wB, h->n * 8, hipMemcpyHostToDevice, s));
   if (h->sv->precision == CRYS_FP32) scft_prep<float>(h);
@@ -926,6 +1014,18 @@ int crys_scft_get_fields(crys_scft_t* h, double* phiA, double* phiB, double* wA,
   if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
   hipStream_t s = h->sv->stream;
   const size_t b = h->n * 8;
+  if (h->m3) {  // full-grid fields, reduced to the 3m sub-lattice
+    std::vector<double> f(h->n);
+    const double* src[4] = {h->phiA, h->phiB, h->wA, h->wB};
+    double* dst[4] = {phiA, phiB, wA, wB};
+    for (int k = 0; k < 4; ++k) {
+      if (!dst[k]) continue;
+      HIPCHK(hipMemcpyAsync(f.data(), src[k], b, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      crys::m3_extract_host(f.data(), dst[k], h->m3->nphy, 1.0);
+    }
+    return CRYS_OK;
+  }
   if (phiA) HIPCHK(hipMemcpyAsync(phiA, h->phiA, b, hipMemcpyDeviceToHost, s));
   if (phiB) HIPCHK(hipMemcpyAsync(phiB, h->phiB, b, hipMemcpyDeviceToHost, s));
   if (wA) HIPCHK(hipMemcpyAsync(wA, h->wA, b, hipMemcpyDeviceToHost, s));
@@ -1004,9 +1104,27 @@ int crys_scft_evaluate(crys_scft_t* h, const double* wA, const double* wB, doubl
   HIPCHK(hipMemcpyAsync(row, h->trace, sizeof(row), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(sums.data(), h->sums, (stress ? 1 + 3 * Ns : 1) * 8,
                         hipMemcpyDeviceToHost, s));
-  if (dH_dwA) HIPCHK(hipMemcpyAsync(dH_dwA, h->gA, n * 8, hipMemcpyDeviceToHost, s));
-  if (dH_dwB) HIPCHK(hipMemcpyAsync(dH_dwB, h->gB, n * 8, hipMemcpyDeviceToHost, s));
+  std::vector<double> gfa, gfb;
+  double* outA = dH_dwA;
+  double* outB = dH_dwB;
+  if (h->m3) {  // gradients on the full grid first
+    if (dH_dwA) {
+      gfa.resize(n);
+      outA = gfa.data();
+    }
+    if (dH_dwB) {
+      gfb.resize(n);
+      outB = gfb.data();
+    }
+  }
+  if (outA) HIPCHK(hipMemcpyAsync(outA, h->gA, n * 8, hipMemcpyDeviceToHost, s));
+  if (outB) HIPCHK(hipMemcpyAsync(outB, h->gB, n * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (h->m3) {
+    // a reduced value sets its 8 images, which share one gradient
+    if (dH_dwA) crys::m3_extract_host(gfa.data(), dH_dwA, h->m3->nphy, 8.0);
+    if (dH_dwB) crys::m3_extract_host(gfb.data(), dH_dwB, h->m3->nphy, 8.0);
+  }
   const double Q = row[CRYS_TRACE_Q];
   if (trace_row) memcpy(trace_row, row, sizeof(row));
   if (H) *H = sums[0] / (double)n - log(Q);

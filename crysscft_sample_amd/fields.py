@@ -95,6 +95,48 @@ def extract(full: np.ndarray, solver: str) -> np.ndarray:
     raise ValueError(solver)
 
 
+def m3_operations(g):
+    """The 8 symmetry operations of a 3m solver with translational part g
+    (generate_m3_operation, solver_utilities.h:55-66): (mirrored axes,
+    translation in cell fractions) for identity, -z, -y, -y-z, -x, -x-z,
+    -x-y, -x-y-z."""
+    g = [float(v) for v in g]
+    return [((), (0.0, 0.0, 0.0)),
+            ((2,), (g[0], g[1], g[2])),
+            ((1,), (g[3], g[4], g[5])),
+            ((1, 2), (g[0] + g[3], -g[1] + g[4], g[2] + g[5])),
+            ((0,), (g[6], g[7], g[8])),
+            ((0, 2), (g[6] - g[0], g[1] + g[7], g[2] + g[8])),
+            ((0, 1), (-g[3] + g[6], g[4] + g[7], g[5] + g[8])),
+            ((0, 1, 2), (-g[0] - g[3] + g[6], -g[1] + g[4] + g[7], g[2] + g[5] + g[8]))]
+
+
+def expand_3m(reduced: np.ndarray, g) -> np.ndarray:
+    """Full-grid field of a 3m solver's reduced field (the even sub-lattice
+    f[2x,2y,2z]) with translational part g: f(op(r)) = f(r) for the 8
+    operations, where on an axis of length N an image index is 2j + tN, or
+    -1 - 2j + tN on a mirrored axis (mod N).  Raises ValueError unless the
+    operations map the sub-lattice one to one onto the grid."""
+    shape = tuple(2 * s for s in reduced.shape)
+    full = np.zeros(shape)
+    hits = np.zeros(shape, dtype=np.int64)
+    idx = np.indices(reduced.shape)
+    for mirrored, t in m3_operations(g):
+        im = []
+        for a in range(3):
+            s = t[a] * shape[a]
+            if abs(s - round(s)) > 1e-9:
+                raise ValueError("translational part is not a whole number of grid points")
+            s = int(round(s))
+            j = 2 * idx[a]
+            im.append(((-1 - j + s) if a in mirrored else (j + s)) % shape[a])
+        full[im[0], im[1], im[2]] = reduced
+        np.add.at(hits, (im[0], im[1], im[2]), 1)
+    if not (hits == 1).all():
+        raise ValueError("the operations do not map the 2x2y2z sub-lattice onto the grid")
+    return full
+
+
 def expand(reduced: np.ndarray, solver: str) -> np.ndarray:
     """Inverse of :func:`extract` for mirror-symmetric fields."""
     if solver == "P1":

@@ -77,9 +77,14 @@ CRYS_API int crys_device_synchronize(void);
  * set_grid() (solver_p1.h:95-125, solver_3m.h:73-113, solver_Pmmm.h:66-96)
  * returning CRYS_ERR_BAD_GRID instead of exit(1).
  * grid_log: logical grid {Nx, Ny, Nz}.  translational_part: 9 doubles for
- * CRYS_3M (glide parts of the -z, -y, -x planes), may be NULL (all zero);
- * ignored for other kinds.  precision: enum crys_precision.  flags: enum
- * crys_flags. */
+ * CRYS_3M (the translations of the -z, -y, -x mirrors in cell fractions, as
+ * generate_m3_operation, solver_utilities.h:55-66), may be NULL (all zero);
+ * ignored for other kinds.  With non-zero parts the solver applies P1's
+ * operators on the logical grid to the field expanded by the 8 operations and
+ * returns the 2x2y2z sub-lattice; a translation that is not a whole number of
+ * grid points, or operations that do not map the sub-lattice one to one onto
+ * the grid, give CRYS_ERR_UNSUPPORTED.  precision: enum crys_precision.
+ * flags: enum crys_flags. */
 CRYS_API int crys_solver_create(int kind, const size_t grid_log[3], const double translational_part[9],
                        int precision, int flags, crys_solver_t** out);
 /* ~CrysPSSolver* (solver_p1.h:55-62, solver_3m.h:55-71, solver_Pmmm.h:54-64). */

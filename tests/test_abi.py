@@ -62,6 +62,19 @@ def test_invalid_arguments():
         C.SolverP1((8, 8, 12))
 
 
+@pytest.mark.parametrize("g,what", [([0.3, 0, 0, 0, 0, 0, 0, 0, 0], b"whole number"),
+                                    ([0, 0, 1 / 16, 0, 0, 0, 0, 0, 0], b"sub-lattice")])
+def test_3m_translational_parts_that_break_the_symmetry_are_rejected(g, what):
+    # checked before any device allocation: no GPU needed
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    grid = (ctypes.c_size_t * 3)(16, 16, 16)
+    gp = (ctypes.c_double * 9)(*g)
+    assert L.crys_solver_create(_lib.M3, grid, gp, 0, 0, ctypes.byref(h)) == _lib.ERR_UNSUPPORTED
+    assert not h.value
+    assert what in L.crys_last_error()
+
+
 def tensor_apply(f, Ds):
     out = np.einsum("ai,ijk->ajk", Ds[0], f)
     out = np.einsum("bj,ajk->abk", Ds[1], out)

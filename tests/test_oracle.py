@@ -201,3 +201,45 @@ def test_three_solvers_same_trace(oracle):
     for name in ("3m", "P1"):
         assert rel_err(traces[name][:, 0], traces["PmmmDCT"][:, 0]) < 1e-12
         assert rel_err(traces[name][:, 5], traces["PmmmDCT"][:, 5]) < 1e-10
+
+
+# ---- 3m with translational parts ----------------------------------------
+# The reference's 3m solver with translational part g (solver_3m.h:172-255,
+# generate_m3_operation solver_utilities.h:55-66) is the P1 diffusion of the
+# symmetric full-grid field: its phase-folded coefficients are pinned by
+# 3m_g(f) == P1(expand_g(f))[::2, ::2, ::2], with P1 pinned above by numpy
+# fixtures.  Mirror-plane offsets (fractional parts on a mirrored axis) fix
+# the direction of the translations: the other orientation fails them.
+M3_G = [
+    [0.5, 0.5, 0, 0, 0, 0, 0, 0, 0],           # n-glide normal to z
+    [0, 0, 0, 0.5, 0, 0.5, 0, 0, 0],           # glide normal to y
+    [0, 0, 0, 0, 0, 0, 0, 0.5, 0.5],           # glide normal to x
+    [0.5, 0.5, 0, 0.5, 0, 0.5, 0, 0.5, 0.5],   # all three
+    [0, 0, 0.25, 0, 0, 0, 0, 0, 0],            # mirror plane at z = 1/4 - h/2
+    [0.5, 0, 0.25, 0, 0.125, 0.5, 0.25, 0, 0],
+]
+
+
+@pytest.mark.parametrize("g", M3_G)
+def test_3m_translational_parts_equal_p1_on_expanded_fields(oracle, g):
+    N, cell, ds = (16, 16, 32), (4.0, 4.3, 4.1), 0.02
+    m3 = oracle.Solver("3m", N, g=g)
+    m3.set_cell_para(cell)
+    m3.set_contour_step(ds)
+    p1 = oracle.Solver("P1", N)
+    p1.set_cell_para(cell)
+    p1.set_contour_step(ds)
+    red = np.random.default_rng(17).standard_normal(m3.grid_size_phy)
+    full = F.expand_3m(red, g)
+    assert rel_err(m3.apply(red), p1.apply(full)[::2, ::2, ::2]) < 1e-13
+    # the expanded result keeps the symmetry
+    out = p1.apply(full)
+    assert rel_err(F.expand_3m(out[::2, ::2, ::2], g), out) < 1e-13
+
+
+def test_3m_expansion_rejects_invalid_operations():
+    red = np.zeros((4, 4, 8))
+    with pytest.raises(ValueError):
+        F.expand_3m(red, [0.3, 0, 0, 0, 0, 0, 0, 0, 0])  # not whole grid points
+    with pytest.raises(ValueError):
+        F.expand_3m(red, [0, 0, 1 / 16, 0, 0, 0, 0, 0, 0])  # maps evens onto evens
