@@ -2,20 +2,24 @@
 // crys_batch_*).
 //
 // Each instance is one diblock_scft_simple_iterate (crysSCFT_sample.cpp:426-
-// 531) with its own grid, chemistry, cell and contour.  One launch of
-// batch_pass_kernel runs one axis pass of one contour step for every active
-// instance and both of its chains: blockIdx.x indexes a work list of
-// (instance, chain, tile) items built on the host, ordered by decreasing
-// contour length so that the instances still running at step t are a prefix
-// of the list.  The per-axis length class (16/32/48/64) of each instance
-// selects the matrix-core tile body at run time.  One SCFT iteration of the
-// whole batch is captured in a hipGraph.
+// 531) with its own grid, chemistry, cell and contour.  When every instance
+// has its three axes in one length class (16/32/48/64), a contour step of
+// the whole batch is one launch of batch_phase_kernel: the alternating
+// x-slab / y-slab phases of plane_phase.hpp, with blockIdx.x indexing a work
+// list of (instance, chain, plane) items, provided that list holds at most
+// 6144 items (small and medium batches).  Otherwise a step is three launches
+// of batch_pass_kernel, one per axis, over (instance, chain, tile) items.
+// Either list is ordered by decreasing contour length so that the instances
+// still running at step t are a prefix of it, and each item's length class
+// selects the matrix-core body at run time.  One SCFT iteration of the whole
+// batch is captured in a hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -27,6 +31,7 @@
 #include "crys_solver.h"
 #include "field_ops.hpp"
 #include "operators.hpp"
+#include "plane_phase.hpp"
 
 namespace crys {
 int set_error(int code, const char* msg);
@@ -35,6 +40,16 @@ int set_error(int code, const char* msg);
 namespace {
 
 constexpr int kCap = 256;  // trace rows kept on the device between copies
+// Above this many (instance, chain, plane) items per phase the batch runs
+// the per-axis passes instead of the one-launch phases; CRYS_BATCH_PHASE_ITEMS
+// overrides it (for A/B timing).
+size_t phase_batch_items() {
+  static const size_t lim = [] {
+    const char* e = getenv("CRYS_BATCH_PHASE_ITEMS");
+    return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)6144;
+  }();
+  return lim;
+}
 
 int errf(int code, const char* fmt, ...) {
   char buf[512];
@@ -72,7 +87,11 @@ struct BatchInst {
   long n;
   const T* mat[3];
   const unsigned short* mat_lo[3];  // FP32: bf16 low parts of the operators
+  const T* mat_c[3];                // half-size centrosymmetric operators (cen)
+  const unsigned short* mat_c_lo[3];
+  int pcls, cen;                    // phase length class (0: none), centrosymmetric
   T *qf, *qb, *scr0, *scr1, *EA, *EB;
+  T *scr0b, *scr1b;                 // second scratch field per chain (phases)
   double *wA, *wB, *ksi, *phiA, *phiB;
   double fA, fB, xN, acc, ds;
 };
@@ -106,6 +125,56 @@ __global__ __launch_bounds__(256) void batch_pass_kernel(const BatchInst<T>* __r
     case 32: mfma_pass_tile<T, AXIS, 32>(ch, mat, lo, I.X, I.Y, I.Z, tidx, lds); break;
     case 48: mfma_pass_tile<T, AXIS, 48>(ch, mat, lo, I.X, I.Y, I.Z, tidx, lds); break;
     default: mfma_pass_tile<T, AXIS, 64>(ch, mat, lo, I.X, I.Y, I.Z, tidx, lds); break;
+  }
+}
+
+// Phase k of the batch (see plane_phase.hpp): work item (instance, chain
+// << 16 | plane); `yslab` selects y-slab phases (odd k).  Forward chain q_f[t]
+// -> q_f[t+1] with E_A for t < NsA; backward chain q_b[Ns-t] -> q_b[Ns-t-1]
+// with E_B while Ns-t-1 >= NsA; phase k writes scratch field k % 2.
+template <typename T>
+__global__ __launch_bounds__(512) void batch_phase_kernel(const BatchInst<T>* __restrict__ insts,
+                                                          const int2* __restrict__ work, int k,
+                                                          int yslab) {
+  __shared__ T lds[64 * PhaseShape<T, 64, false, 512>::PB];
+  const int2 w = work[blockIdx.x];
+  const BatchInst<T>& I = insts[w.x];
+  const int chain = w.y >> 16, plane = w.y & 0xffff;
+  const long n = I.n;
+  const int Ns = I.Ns, NsA = I.NsA;
+  T* q = chain == 0 ? I.qf : I.qb;
+  T* half[2] = {chain == 0 ? I.scr0 : I.scr1, chain == 0 ? I.scr0b : I.scr1b};
+  auto E = [&](int t) -> const T* {
+    return chain == 0 ? (t < NsA ? I.EA : I.EB) : (Ns - t - 1 >= NsA ? I.EB : I.EA);
+  };
+  PhaseChain<T> ch{};
+  ch.in = k == 0 ? (chain == 0 ? q : q + (long)Ns * n) : half[(k + 1) % 2];
+  if (k >= 1) {
+    ch.hist = chain == 0 ? q + (long)k * n : q + (long)(Ns - k) * n;
+    ch.e_fin = E(k - 1);
+  }
+  if (k < Ns) {
+    ch.out = half[k % 2];
+    ch.e_next = E(k);
+  }
+  const int ra = yslab ? 0 : 1;
+  const int MR = yslab ? I.X : I.Y, MC = I.Z;
+  const long base = yslab ? (long)plane * I.Z : (long)plane * I.Y * I.Z;
+  const long RS = yslab ? (long)I.Y * I.Z : (long)I.Z;
+  if (I.cen) {
+    const T *mr = I.mat_c[ra], *mc = I.mat_c[2];
+    const unsigned short *lr = I.mat_c_lo[ra], *lc = I.mat_c_lo[2];
+    if (I.pcls == 32) plane_phase_body<T, 32, true, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
+    else plane_phase_body<T, 64, true, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
+    return;
+  }
+  const T *mr = I.mat[ra], *mc = I.mat[2];
+  const unsigned short *lr = I.mat_lo[ra], *lc = I.mat_lo[2];
+  switch (I.pcls) {
+    case 16: plane_phase_body<T, 16, false, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
+    case 32: plane_phase_body<T, 32, false, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
+    case 48: plane_phase_body<T, 48, false, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
+    default: plane_phase_body<T, 64, false, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
   }
 }
 
@@ -158,19 +227,28 @@ __global__ __launch_bounds__(kRedThreads) void batch_phi_kernel(const BatchInst<
   const int NsA = I.NsA, NsB = I.NsB;
   double s_e1 = 0.0, s_enth = 0.0, s_fk = 0.0, m_inc = 0.0;
   for (long i = (long)w.y * kRedThreads + threadIdx.x; i < n; i += (long)I.nblk * kRedThreads) {
-    double pA = 0.0;
-    for (int s = 0; s <= NsA; ++s) {
-      double factor = s == 0 ? 0.5 : (s == NsA ? 0.5 : 1);
-      factor *= ca;
-      pA += factor * (double)I.qf[s * n + i] * (double)I.qb[s * n + i];
-    }
-    double pB = 0.0;
-    for (int s = 0; s <= NsB; ++s) {
-      double factor = s == 0 ? 0.5 : (s == NsB ? 0.5 : 1);
-      factor *= cb;
-      const long k = (NsA + s) * n + i;
-      pB += factor * (double)I.qf[k] * (double)I.qb[k];
-    }
+    // trapezoid sum over slots s0..s1, four slots per round (as
+    // phi_energy_mix_kernel)
+    auto contour_sum = [&](long s0, long s1) -> double {
+      double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+      long s = s0 + 1;
+      for (; s + 4 <= s1; s += 4) {
+        const T a0 = I.qf[s * n + i], a1 = I.qf[(s + 1) * n + i];
+        const T a2 = I.qf[(s + 2) * n + i], a3 = I.qf[(s + 3) * n + i];
+        const T b0 = I.qb[s * n + i], b1 = I.qb[(s + 1) * n + i];
+        const T b2 = I.qb[(s + 2) * n + i], b3 = I.qb[(s + 3) * n + i];
+        p0 += (double)a0 * (double)b0;
+        p1 += (double)a1 * (double)b1;
+        p2 += (double)a2 * (double)b2;
+        p3 += (double)a3 * (double)b3;
+      }
+      for (; s < s1; ++s) p0 += (double)I.qf[s * n + i] * (double)I.qb[s * n + i];
+      const double ends = 0.5 * ((double)I.qf[s0 * n + i] * (double)I.qb[s0 * n + i] +
+                                 (double)I.qf[s1 * n + i] * (double)I.qb[s1 * n + i]);
+      return (p0 + p1) + (p2 + p3) + ends;
+    };
+    const double pA = ca * contour_sum(0, NsA);
+    const double pB = cb * contour_sum(NsA, NsA + NsB);
     const double a = I.wA[i], b = I.wB[i], x = I.ksi[i];
     s_e1 += -a * pA - b * pB;
     s_enth += xN * pA * pB;
@@ -247,6 +325,10 @@ struct crys_batch {
   std::vector<size_t> n, NsA, NsB, off_f, off_q;  // per instance sizes and offsets
   std::vector<int> cls;                           // 3 per instance
   std::vector<size_t> off_m;                      // matrix offsets, 3 per instance
+  std::vector<size_t> off_mc;                     // half-size matrix offsets (cen instances)
+  bool phases = false;                            // one launch per step (batch_phase_kernel)
+  int2* pwork[2] = {nullptr, nullptr};            // phase items: x-slab, y-slab phases
+  std::vector<int> pprefix[2];                    // phase items active at phase k
   hipStream_t stream = nullptr;
   void *dev = nullptr;  // one allocation for every buffer
   void* insts = nullptr;
@@ -262,6 +344,8 @@ struct crys_batch {
   // typed base pointers
   void *qf = nullptr, *qb = nullptr, *scr = nullptr, *EA = nullptr, *EB = nullptr, *mats = nullptr;
   unsigned short* mats_lo = nullptr;  // FP32: bf16 low parts of the operators
+  void* mats_c = nullptr;             // half-size centrosymmetric operators
+  unsigned short* mats_c_lo = nullptr;
   double *wA = nullptr, *wB = nullptr, *ksi = nullptr, *phiA = nullptr, *phiB = nullptr;
 };
 
@@ -294,7 +378,36 @@ int balloc(crys_batch* b, void** p, size_t bytes) {
 template <typename T>
 int batch_setup(crys_batch* b) {
   const size_t C = b->count;
-  size_t tot_f = 0, tot_q = 0, tot_m = 0;
+  size_t tot_f = 0, tot_q = 0, tot_m = 0, tot_mc = 0;
+  // the one-launch phases need every instance's three axes in one class;
+  // centrosymmetric products where all three lengths equal a class of 32/64
+  std::vector<int> pcls(C, 0), cen(C, 0);
+  bool all = crys::phase_enabled();
+  for (size_t i = 0; i < C; ++i) {
+    const crys_instance_t& d = b->desc[i];
+    const int red = d.kind == CRYS_P1 ? 1 : 2;
+    const int c = b->cls[3 * i];
+    if (b->cls[3 * i + 1] == c && b->cls[3 * i + 2] == c) pcls[i] = c;
+    else all = false;
+    bool ce = pcls[i] != 0 && c % 32 == 0 && d.kind != CRYS_3M && crys::centro_enabled();
+    for (int a = 0; a < 3; ++a) ce = ce && (int)(d.grid_log[a] / red) == c;
+    cen[i] = ce ? 1 : 0;
+  }
+  // Measured on MI355X (PmmmDCT instances of 16^3..64^3, FP64 / FP32
+  // phases over passes): 1.57 / 2.11 at 176 items, 1.52 / 1.71 at 656,
+  // 1.21 / 1.16 at 2240, 1.11 / 1.01 at 4752, 0.99 / 0.90 at 9296 and
+  // 0.93 / 0.92 at 18400, where the passes' lighter workgroups keep more
+  // waves in flight.
+  size_t items = 0;
+  for (size_t i = 0; i < C; ++i) {
+    const crys_instance_t& d = b->desc[i];
+    const size_t red = d.kind == CRYS_P1 ? 1 : 2;
+    items += 2 * std::max(d.grid_log[0], d.grid_log[1]) / red;
+  }
+  all = all && items <= phase_batch_items();
+  b->phases = all;
+  if (!all) std::fill(cen.begin(), cen.end(), 0);
+  b->off_mc.assign(3 * C, 0);
   for (size_t i = 0; i < C; ++i) {
     b->off_f[i] = tot_f;
     b->off_q[i] = tot_q;
@@ -305,11 +418,16 @@ int batch_setup(crys_batch* b) {
       b->off_m[3 * i + a] = tot_m;
       const size_t M = d.kind == CRYS_P1 ? d.grid_log[a] : d.grid_log[a] / 2;
       tot_m += M * M;
+      if (cen[i]) {
+        b->off_mc[3 * i + a] = tot_mc;
+        tot_mc += M * M / 2;
+      }
     }
   }
   BCHK(balloc(b, &b->qf, tot_q * sizeof(T)));
   BCHK(balloc(b, &b->qb, tot_q * sizeof(T)));
-  BCHK(balloc(b, &b->scr, 2 * tot_f * sizeof(T)));
+  // one scratch field per chain; the phases alternate between two
+  BCHK(balloc(b, &b->scr, (b->phases ? 4 : 2) * tot_f * sizeof(T)));
   BCHK(balloc(b, &b->EA, tot_f * sizeof(T)));
   BCHK(balloc(b, &b->EB, tot_f * sizeof(T)));
   BCHK(balloc(b, &b->mats, tot_m * sizeof(T)));
@@ -321,8 +439,9 @@ int batch_setup(crys_batch* b) {
   BCHK(balloc(b, (void**)&b->trace, C * kCap * 7 * 8));
   BCHK(balloc(b, (void**)&b->counter, sizeof(int)));
 
-  // operators
+  // operators (and, for the half-size forms, their double values)
   std::vector<T> mats(tot_m);
+  std::vector<double> mats_d(tot_mc ? tot_m : 0);
   std::vector<unsigned short> lo(std::is_same<T, float>::value ? tot_m : 0);
   for (size_t i = 0; i < C; ++i) {
     const crys_instance_t& d = b->desc[i];
@@ -337,6 +456,7 @@ int batch_setup(crys_batch* b) {
       else if (d.kind == CRYS_P1) At = crys::p1_axis_operator(Nl, L, d.ds);
       else At = crys::m3_axis_operator(Nl, L, d.ds);
       std::copy(At.begin(), At.end(), mats.begin() + b->off_m[3 * i + a]);
+      if (!mats_d.empty()) std::copy(At.begin(), At.end(), mats_d.begin() + b->off_m[3 * i + a]);
       if (std::is_same<T, float>::value)
         for (size_t k = 0; k < At.size(); ++k)
           lo[b->off_m[3 * i + a] + k] =
@@ -347,6 +467,33 @@ int batch_setup(crys_batch* b) {
   if (std::is_same<T, float>::value) {
     BCHK(balloc(b, (void**)&b->mats_lo, tot_m * sizeof(unsigned short)));
     BHIP(hipMemcpy(b->mats_lo, lo.data(), tot_m * sizeof(unsigned short), hipMemcpyHostToDevice));
+  }
+  if (tot_mc) {  // half-size forms (centro_half) of the cen instances' operators
+    std::vector<T> mc(tot_mc);
+    std::vector<unsigned short> mclo(std::is_same<T, float>::value ? tot_mc : 0);
+    for (size_t i = 0; i < C; ++i) {
+      if (!cen[i]) continue;
+      const crys_instance_t& d = b->desc[i];
+      for (int a = 0; a < 3; ++a) {
+        const int M = (int)(d.kind == CRYS_P1 ? d.grid_log[a] : d.grid_log[a] / 2);
+        const size_t o = b->off_m[3 * i + a];
+        std::vector<double> At(M * (size_t)M);
+        for (size_t k = 0; k < At.size(); ++k) At[k] = (double)mats_d[o + k];
+        const std::vector<double> Cm = crys::centro_half(At, M);
+        std::copy(Cm.begin(), Cm.end(), mc.begin() + b->off_mc[3 * i + a]);
+        if (std::is_same<T, float>::value)
+          for (size_t k = 0; k < Cm.size(); ++k)
+            mclo[b->off_mc[3 * i + a] + k] =
+                crys::bf16_bits((float)(Cm[k] - (double)(float)Cm[k]));
+      }
+    }
+    BCHK(balloc(b, &b->mats_c, tot_mc * sizeof(T)));
+    BHIP(hipMemcpy(b->mats_c, mc.data(), tot_mc * sizeof(T), hipMemcpyHostToDevice));
+    if (std::is_same<T, float>::value) {
+      BCHK(balloc(b, (void**)&b->mats_c_lo, tot_mc * sizeof(unsigned short)));
+      BHIP(hipMemcpy(b->mats_c_lo, mclo.data(), tot_mc * sizeof(unsigned short),
+                     hipMemcpyHostToDevice));
+    }
   }
 
   // instance table
@@ -365,7 +512,11 @@ int batch_setup(crys_batch* b) {
       I.cls[a] = b->cls[3 * i + a];
       I.mat[a] = (const T*)b->mats + b->off_m[3 * i + a];
       I.mat_lo[a] = b->mats_lo ? b->mats_lo + b->off_m[3 * i + a] : nullptr;
+      I.mat_c[a] = cen[i] ? (const T*)b->mats_c + b->off_mc[3 * i + a] : nullptr;
+      I.mat_c_lo[a] = (cen[i] && b->mats_c_lo) ? b->mats_c_lo + b->off_mc[3 * i + a] : nullptr;
     }
+    I.pcls = pcls[i];
+    I.cen = cen[i];
     I.NsA = (int)b->NsA[i];
     I.NsB = (int)b->NsB[i];
     I.Ns = I.NsA + I.NsB;
@@ -374,7 +525,9 @@ int batch_setup(crys_batch* b) {
     I.qf = qf + b->off_q[i];
     I.qb = qb + b->off_q[i];
     I.scr0 = scr + b->off_f[i];
-    I.scr1 = scr + (b->off_f[i] + b->off_f.back() + b->n.back());
+    I.scr1 = scr + (b->off_f[i] + tot_f);
+    I.scr0b = b->phases ? scr + (b->off_f[i] + 2 * tot_f) : nullptr;
+    I.scr1b = b->phases ? scr + (b->off_f[i] + 3 * tot_f) : nullptr;
     I.EA = (T*)b->EA + b->off_f[i];
     I.EB = (T*)b->EB + b->off_f[i];
     I.wA = b->wA + b->off_f[i];
@@ -420,6 +573,29 @@ int batch_setup(crys_batch* b) {
     BCHK(balloc(b, (void**)&b->work[a], wl.size() * sizeof(int2)));
     BHIP(hipMemcpy(b->work[a], wl.data(), wl.size() * sizeof(int2), hipMemcpyHostToDevice));
   }
+  if (b->phases) {
+    for (int ys = 0; ys < 2; ++ys) {
+      std::vector<int2> wl;
+      std::vector<int> end_of;
+      for (size_t k = 0; k < C; ++k) {
+        const size_t i = order[k];
+        const int planes = ys ? tab[i].Y : tab[i].X;
+        for (int ch = 0; ch < 2; ++ch)
+          for (int pl = 0; pl < planes; ++pl) wl.push_back(int2{(int)i, (ch << 16) | pl});
+        end_of.push_back((int)wl.size());
+      }
+      // phase k (0..Ns) of an instance with Ns contour steps
+      b->pprefix[ys].assign(b->Ns_max + 1, 0);
+      for (int k = 0; k <= b->Ns_max; ++k) {
+        int cnt = 0;
+        for (size_t j = 0; j < C; ++j)
+          if ((int)(b->NsA[order[j]] + b->NsB[order[j]]) >= k) cnt = end_of[j];
+        b->pprefix[ys][k] = cnt;
+      }
+      BCHK(balloc(b, (void**)&b->pwork[ys], wl.size() * sizeof(int2)));
+      BHIP(hipMemcpy(b->pwork[ys], wl.data(), wl.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+  }
   std::vector<int2> rw;
   for (size_t i = 0; i < C; ++i)
     for (int k = 0; k < tab[i].nblk; ++k) rw.push_back(int2{(int)i, k});
@@ -443,7 +619,15 @@ int batch_setup(crys_batch* b) {
 template <typename T>
 int batch_record(crys_batch* b, hipStream_t s) {
   const crys::BatchInst<T>* I = (const crys::BatchInst<T>*)b->insts;
-  for (int t = 0; t < b->Ns_max; ++t) {
+  for (int k = 0; b->phases && k <= b->Ns_max; ++k) {
+    const int ys = k % 2;
+    const int cnt = b->pprefix[ys][k];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL((crys::batch_phase_kernel<T>), dim3(cnt), dim3(512), 0, s, I,
+                       (const int2*)b->pwork[ys], k, ys);
+    BHIP(hipGetLastError());
+  }
+  for (int t = 0; !b->phases && t < b->Ns_max; ++t) {
     const int ax_order[3] = {2, 1, 0};
     for (int a : ax_order) {
       const int cnt = b->prefix[a][t];

@@ -83,6 +83,16 @@ bool phase_half_enabled() {
   return on;
 }
 
+// CRYS_CENTRO=0 uses the dense operators instead of the centrosymmetric
+// half-size products (for A/B timing).
+bool centro_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("CRYS_CENTRO");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
 // CRYS_SLAB=0 runs the z and y passes as two launches (for A/B timing).
 bool slab_enabled() {
   static const bool on = [] {
@@ -169,13 +179,6 @@ void free_ops(crys_solver* h) {
   h->cur = -1;
 }
 
-bool centro_enabled() {
-  static const bool on = [] {
-    const char* e = getenv("CRYS_CENTRO");
-    return !(e && strcmp(e, "0") == 0);
-  }();
-  return on;
-}
 
 // Uploads one operator in the solver's precision (and, for FP32, the bf16
 // bits of its low part to *lo when lo is not null).
@@ -211,7 +214,7 @@ int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3
     else if (h->kind == CRYS_P1) At = crys::p1_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
     else At = crys::m3_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
     CHK(upload_matrix(h, At, &dst[a], lo ? &lo[a] : nullptr));
-    if (cen && h->kind != CRYS_3M && M % 32 == 0 && M <= 64 && centro_enabled()) {
+    if (cen && h->kind != CRYS_3M && M % 32 == 0 && M <= 64 && crys::centro_enabled()) {
       std::vector<double> C = crys::centro_half(At, M);
       CHK(upload_matrix(h, C, &cen[a], cen_lo ? &cen_lo[a] : nullptr));
     }

@@ -180,27 +180,29 @@ __device__ __forceinline__ void phase_col_product(
   }
 }
 
-template <typename T, int MP, bool CEN, int NT_ = 512>
-__global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
+// One phase of one chain on one plane (base: its first element; RS: its row
+// stride), for the single-field and the batched kernels.  `lds` holds at
+// least MP * PhaseShape::PB elements.
+template <typename T, int MP, bool CEN, int NT_>
+__device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const T* mat_r,
+                                                 const T* mat_c,
+                                                 const unsigned short* mat_r_lo,
+                                                 const unsigned short* mat_c_lo, int MR, int MC,
+                                                 long base, long RS, T* lds) {
   using S = PhaseShape<T, MP, CEN, NT_>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int NT = S::NT, JT = S::JT, H = S::H, PB = S::PB, PA = S::PA, TPW = S::TPW;
   constexpr bool SPLIT = std::is_same<T, float>::value;
-  __shared__ T lds[MP * PB];
 
-  const PhaseChain<T> ch = p.c[blockIdx.y];
-  const int MR = p.MR, MC = p.MC;
-  const long RS = p.row_stride;
-  const long base = (long)blockIdx.x * p.plane_stride;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
-  const bool lo_r = SPLIT && p.lo_r != nullptr, lo_c = SPLIT && p.lo_c != nullptr;
+  const bool lo_r = SPLIT && mat_r_lo != nullptr, lo_c = SPLIT && mat_c_lo != nullptr;
 
   // operators: the row tile w % JT of the row operator, the output column
   // tile w % JT of the z operator
   SlabOperands<T, MP, CEN> orow, ocol;
-  slab_load_operands<T, MP, CEN>(p.mat_r, p.lo_r, MR, w % JT, lane, orow);
-  slab_load_operands<T, MP, CEN>(p.mat_c, p.lo_c, MC, w % JT, lane, ocol);
+  slab_load_operands<T, MP, CEN>(mat_r, mat_r_lo, MR, w % JT, lane, orow);
+  slab_load_operands<T, MP, CEN>(mat_c, mat_c_lo, MC, w % JT, lane, ocol);
 
   // ---- every global load up front: the plane and the E values that the
   // E/history stage needs (same element per task), one memory latency ----
@@ -395,6 +397,13 @@ __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
         __builtin_nontemporal_store(lds[r * PB + c], &ch.out[base + r * RS + c]);
     }
   }
+}
+
+template <typename T, int MP, bool CEN, int NT_ = 512>
+__global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
+  __shared__ T lds[MP * PhaseShape<T, MP, CEN, NT_>::PB];
+  plane_phase_body<T, MP, CEN, NT_>(p.c[blockIdx.y], p.mat_r, p.mat_c, p.lo_r, p.lo_c, p.MR, p.MC,
+                                    (long)blockIdx.x * p.plane_stride, p.row_stride, lds);
 }
 
 // One 16 x 16 output tile with the plane as the B operand: out[j][c] =
@@ -741,6 +750,7 @@ inline int phase_class(int X, int Y, int Z) {
 }
 
 bool phase_enabled();
+bool centro_enabled();
 
 // Launches one phase for 1 or 2 chains.  `cen` selects the centrosymmetric
 // kernel (then the caller passes the half-size operators and MR == MC == mp,

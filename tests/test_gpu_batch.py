@@ -91,3 +91,37 @@ def test_random_batch_fp32_error_against_fp64():
     dF = np.abs(out["fp32"][:, :, 0] / out["fp64"][:, :, 0] - 1).max(axis=1)
     say(f"  fp32 batch: per-instance max |dF/F| median {np.median(dF):.2e} max {dF.max():.2e}")
     assert dF.max() < 1e-5
+
+
+# Every instance with its three reduced axes in one length class: the batch
+# runs one launch per contour step (batch_phase_kernel) over all instances.
+PHASE_BATCH = [
+    Instance("PmmmDCT", (32, 32, 32), (4.0,) * 3, 0.25, 20.0, 0.01),        # class 16
+    Instance("PmmmDCT", (64, 64, 64), (4.2,) * 3, 0.3, 25.0, 1 / 150),       # 32, centrosymmetric
+    Instance("P1", (32, 32, 32), (4.1,) * 3, 0.28, 22.0, 1 / 120),           # 32, centrosymmetric
+    Instance("3m", (32, 32, 32), (3.9,) * 3, 0.2, 30.0, 0.01),               # 16 (3m: dense)
+    Instance("PmmmDCT", (96, 96, 96), (4.4,) * 3, 0.31, 24.0, 0.02),         # 48
+    Instance("PmmmDCT", (128, 128, 128), (4.2,) * 3, 0.3, 25.0, 0.01),       # 64, centrosymmetric
+    Instance("P1", (40, 40, 40), (4.0,) * 3, 0.33, 18.0, 1 / 80),            # 48, padded
+]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_phase_batch_matches_oracle_per_instance(oracle, prec):
+    nstep = 3
+    b = Batch(PHASE_BATCH, precision=prec)
+    ws = [seeded_fields(inst, 400 + k) for k, inst in enumerate(PHASE_BATCH)]
+    for k, (wA, wB) in enumerate(ws):
+        b.set_fields(k, wA, wB)
+    tr = b.run(nstep)
+    for k, inst in enumerate(PHASE_BATCH):
+        ref = oracle.Solver(inst.solver, inst.grid_log)
+        t_ref, *_ = ref.scft(inst.fA, inst.ds, inst.xN, inst.cell, inst.acceptance, nstep, *ws[k])
+        dF = np.abs(tr[k, :, 0] / t_ref[:, 0] - 1).max()
+        dM = np.abs(tr[k, :, 5] / t_ref[:, 5] - 1).max()
+        say(f"phase batch {prec} instance {k} {inst.solver} {inst.grid_log}: "
+            f"|dF/F| {dF:.2e} |dMaxInc| {dM:.2e}")
+        if prec == "fp64":
+            assert dF <= 1e-10 and dM <= 1e-10
+        else:
+            assert dF < 1e-5
