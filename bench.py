@@ -73,8 +73,10 @@ def run_gpu(precision, steps, warmup, wA, wB):
     C.lib().crys_device_synchronize()
     dt = time.perf_counter() - t0
     ms_step, ms_launch = run.time_step(reps=200)
+    phiA, phiB = run.fields()[:2]
     out = dict(iter_s=steps / dt, ms_per_iter=dt / steps * 1e3, trace=tr, warm=warm, NsA=run.NsA,
-               NsB=run.NsB, ms_per_contour_step=ms_step, ms_per_launch=ms_launch)
+               NsB=run.NsB, ms_per_contour_step=ms_step, ms_per_launch=ms_launch,
+               phiA=phiA, phiB=phiB)
     run.close()
     solver.close()
     return out
@@ -161,23 +163,33 @@ def batch_timing(count, steps):
                      "Ns in {100,150,200}",
            "unit": "instance-iterations/s", "higher_is_better": True, "instances": count,
            "steps": steps, "points_total": 0}
+    fields = None
+    traces, phis = {}, {}
     for prec in ("fp64", "fp32"):
         b = Batch(insts, precision=prec)
         out["points_total"] = int(sum(i[0] for i in b.info))
-        for k, inst in enumerate(insts):
-            sh = b.shape(k)
-            wA = inst.xN * 0.1 * rng.standard_normal(sh)
+        if fields is None:
+            fields = [inst.xN * 0.1 * rng.standard_normal(b.shape(k)) for k, inst in enumerate(insts)]
+        for k, wA in enumerate(fields):
             b.set_fields(k, wA, -wA)
-        b.run(1)  # warm: graph capture
+        warm = b.run(1)  # warm: graph capture
         t0 = time.perf_counter()
         tr = b.run(steps)
         dt = time.perf_counter() - t0
         if not np.isfinite(tr[:, :, 0]).all():
             raise RuntimeError("non-finite free energy in the batch")
+        traces[prec] = np.concatenate([warm, tr], axis=1)
+        phis[prec] = [b.fields(k)[0] for k in range(count)]
         out[prec] = {"instance_iter_s": round(count * steps / dt, 2),
                      "ms_per_batch_iteration": round(dt / steps * 1e3, 3)}
         b.close()
         print(f"bench: batch {prec} done", file=sys.stderr, flush=True)
+    # per-instance error of the FP32 propagators against the FP64 batch, over
+    # every iteration from the same starting fields
+    dF = np.abs(traces["fp32"][:, :, 0] / traces["fp64"][:, :, 0] - 1).max(axis=1)
+    dphi = np.array([np.abs(a - b).max() for a, b in zip(phis["fp32"], phis["fp64"])])
+    out["fp32_vs_fp64"] = {"max_rel_dF": float(dF.max()), "median_rel_dF": float(np.median(dF)),
+                           "max_abs_dphiA": float(dphi.max())}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -286,6 +298,10 @@ def main():
                          "speedup_vs_fp64": round(mixed["iter_s"] / f64["iter_s"], 3),
                          "hbm_gbs": round(b_iter32 * mixed["iter_s"] / 1e9, 2),
                          "max_rel_dF_vs_fp64_gpu": dF,
+                         "max_rel_dMaxInc_vs_fp64_gpu": float(np.abs(
+                             mixed["trace"][:, 5] / f64["trace"][:, 5] - 1).max()),
+                         "max_abs_dphiA_vs_fp64_gpu": float(np.abs(mixed["phiA"] - f64["phiA"]).max()),
+                         "max_abs_dphiB_vs_fp64_gpu": float(np.abs(mixed["phiB"] - f64["phiB"]).max()),
                          "ms_per_contour_step": round(mixed["ms_per_contour_step"], 6),
                          "ms_per_launch": {nm.replace("double", "float"): round(t, 6)
                                            for nm, t in zip(names, mixed["ms_per_launch"])}}

@@ -80,6 +80,9 @@ SIGNATURES = {
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.POINTER(_c_vp)]),
     "crys_scft_destroy": (ctypes.c_int, [_c_vp]),
+    "crys_scft_set_checkpointing": (ctypes.c_int, [_c_vp, _c_sz, _c_sz]),
+    "crys_scft_get_checkpointing": (ctypes.c_int, [_c_vp, ctypes.POINTER(_c_sz),
+                                                   ctypes.POINTER(_c_sz)]),
     "crys_scft_set_fields": (ctypes.c_int, [_c_vp, _c_dp, _c_dp]),
     "crys_scft_run": (ctypes.c_int, [_c_vp, _c_sz, _c_dp]),
     "crys_scft_get_fields": (ctypes.c_int, [_c_vp, _c_dp, _c_dp, _c_dp, _c_dp]),

@@ -156,6 +156,13 @@ struct crys_scft {
   double *qpart = nullptr, *epart = nullptr, *scal = nullptr, *trace = nullptr;
   double *gA = nullptr, *gB = nullptr, *spart = nullptr, *sums = nullptr;  // gradient, lazily
   int* counter = nullptr;
+  // Contour storage, allocated on first use.  every == 0: qf and qb hold all
+  // Ns+1 slots of each chain.  every > 0: ck holds the forward chain at the
+  // slots 0, every, 2 every, ... (nck of them), and qf / qb hold one segment
+  // of every+1 slots that is recomputed from its checkpoint.
+  size_t every_req = 0, budget = 0;  // as requested (0: automatic, 0: free memory)
+  size_t every = 0, nck = 0;
+  void* ck = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const void* graph_mats[3] = {nullptr, nullptr, nullptr};
@@ -295,9 +302,14 @@ int phase_mp(const crys_solver* h) {
 // (row axis y), odd phases on y-slabs (row axis x).  Forward chain: q_f[t]
 // -> q_f[t+1] with E_A for t < NsA; backward chain: q_b[Ns-t] -> q_b[Ns-t-1]
 // with E_B while Ns-t-1 >= NsA.
+//
+// The chains may also cover a part of the contour: qf / qb then hold slots
+// t0 .. t0+Ns (a checkpoint segment), NsA stays the global junction slot, and
+// nch = 1 advances the forward chain alone.
 template <typename T>
 int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns, size_t NsA,
-                      T* qf, T* qb, T* const scr[2], const T* EA, const T* EB, hipStream_t s) {
+                      T* qf, T* qb, T* const scr[2], const T* EA, const T* EB, hipStream_t s,
+                      int nch = 2, size_t t0 = 0) {
   const int X = (int)h->nphy[0], Y = (int)h->nphy[1], Z = (int)h->nphy[2];
   const size_t n = (size_t)X * Y * Z;
   const int mp = phase_mp(h);
@@ -314,8 +326,8 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
   p.plane_stride = yslab ? (long)Z : (long)Y * Z;
   p.row_stride = yslab ? (long)Y * Z : (long)Z;
   const int nplanes = yslab ? Y : X;
-  auto Ef = [&](size_t t) { return t < NsA ? EA : EB; };
-  auto Eb = [&](size_t t) { return Ns - t - 1 >= NsA ? EB : EA; };
+  auto Ef = [&](size_t t) { return t0 + t < NsA ? EA : EB; };
+  auto Eb = [&](size_t t) { return t0 + Ns - t - 1 >= NsA ? EB : EA; };
   crys::PhaseChain<T>& f = p.c[0];
   crys::PhaseChain<T>& b = p.c[1];
   // phase k writes scratch half k % 2 and reads the half phase k-1 wrote
@@ -334,7 +346,7 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
     b.out = scr[1] + wr;
     b.e_next = Eb(k);
   }
-  HIPCHK(crys::launch_phase<T>(p, nplanes, 2, mp, cen, s));
+  HIPCHK(crys::launch_phase<T>(p, nplanes, nch, mp, cen, s));
   return CRYS_OK;
 }
 
@@ -739,36 +751,135 @@ void scft_free(crys_scft* h) {
   void* bufs[] = {h->wA,    h->wB,   h->ksi,   h->phiA,    h->phiB,   h->EA,
                   h->EB,    h->qf,   h->qb,    h->scr[0],  h->scr[1], h->qpart,
                   h->epart, h->scal, h->trace, h->counter, h->gA,     h->gB,
-                  h->spart, h->sums};
+                  h->spart, h->sums,  h->ck};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+}
+
+// Advances the chains over c contour steps of a segment whose slot 0 is the
+// global slot t0: q_f[k] -> q_f[k+1] in qf and q_b[c-k] -> q_b[c-k-1] in qb,
+// from qf[0] and qb[c] (nch = 1: the forward chain alone).  The whole
+// contour is the segment t0 = 0, c = Ns.
+template <typename T>
+int propagate_segment(crys_scft* h, hipStream_t s, int nch, T* qf, T* qb, size_t c, size_t t0) {
+  const crys_solver* sv = h->sv;
+  const OpSet& op = sv->ops[sv->cur];
+  const size_t n = h->n, NsA = h->NsA;
+  T* scr[2] = {(T*)h->scr[0], (T*)h->scr[1]};
+  const T* EA = (const T*)h->EA;
+  const T* EB = (const T*)h->EB;
+  if (phase_mp(sv)) {
+    for (size_t k = 0; k <= c; ++k)
+      CHK(launch_phase_step<T>(sv, op, k, c, NsA, qf, qb, scr, EA, EB, s, nch, t0));
+  } else {
+    for (size_t t = 0; t < c; ++t) {
+      const size_t bo = c - t - 1;
+      const T* in[2] = {qf + t * n, qb + (c - t) * n};
+      T* out[2] = {qf + (t + 1) * n, qb + bo * n};
+      const T* E[2] = {t0 + t < NsA ? EA : EB, t0 + bo >= NsA ? EB : EA};
+      CHK(launch_step<T>(sv, op, nch, in, out, scr, E, s));
+    }
+  }
+  return CRYS_OK;
+}
+
+// d Q / d L_a for the contour steps t_begin .. t_end-1: for every step t and
+// axis a, < E_t q^dagger_{t+1}, (dD_a (x) D_b (x) D_c) E_t q_t > with the
+// derivative operator on axis a.  qf and qb hold the slots from slot0 on.
+// Partials go to h->spart[(3 t + a) * nb + block].
+template <typename T>
+int scft_record_stress(crys_scft* h, hipStream_t s, const T* qf, const T* qb, size_t t_begin,
+                       size_t t_end, size_t slot0) {
+  const crys_solver* sv = h->sv;
+  const OpSet& op = sv->ops[sv->cur];
+  const size_t n = h->n, NsA = h->NsA;
+  T* w = (T*)h->scr[0];
+  const int nb = crys::red_blocks(n);
+  crys::PassParams<T> p{};
+  p.X = (int)sv->nphy[0];
+  p.Y = (int)sv->nphy[1];
+  p.Z = (int)sv->nphy[2];
+  for (size_t t = t_begin; t < t_end; ++t) {
+    const T* E = (const T*)(t < NsA ? h->EA : h->EB);
+    const T* q = qf + (t - slot0) * n;
+    const T* qd = qb + (t + 1 - slot0) * n;
+    for (int a = 0; a < 3; ++a) {
+      // z pass (pre E), then y, then x; the axis a pass uses dD/dL_a
+      const int order[3] = {2, 1, 0};
+      for (int k = 0; k < 3; ++k) {
+        const int ax = order[k];
+        p.c[0] = {k == 0 ? q : w, w, k == 0 ? E : nullptr, nullptr};
+        p.mat = (const T*)(ax == a ? op.dmat_dL[ax] : op.dmat[ax]);
+        p.mat_lo = ax == a ? nullptr : (const unsigned short*)op.dmat_lo[ax];
+        HIPCHK(crys::launch_axis_pass<T>(ax, p, 1, s));
+      }
+      hipLaunchKernelGGL((crys::dot3_partial_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
+                         (const T*)w, E, qd, h->spart + (3 * t + a) * nb, n);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  return CRYS_OK;
+}
+
+// The propagation and contour sums of one iteration with checkpoints every
+// C = h->every slots (sqrt(Ns) checkpointing): a forward sweep keeps q_f at
+// the slots j C; then, last segment first, each segment [s0, s1] recomputes
+// q_f from its checkpoint while the backward chain advances over it, and
+// phi_accumulate_kernel adds the segment's slots to the contour sums (held in
+// phiA / phiB).  One propagation of the forward chain more than the plain
+// iteration; storage nck + 2 (C + 1) fields instead of 2 (Ns + 1).
+template <typename T>
+int record_checkpointed(crys_scft* h, hipStream_t s, bool stress) {
+  const size_t n = h->n, Ns = h->Ns, C = h->every, J = h->nck;
+  T* F = (T*)h->qf;
+  T* B = (T*)h->qb;
+  T* ck = (T*)h->ck;
+  const unsigned g = grid_for(n);
+  auto copy = [&](T* dst, const T* src) -> int {
+    hipLaunchKernelGGL((crys::copy_kernel<T>), dim3(g), dim3(256), 0, s, dst, src, n);
+    HIPCHK(hipGetLastError());
+    return CRYS_OK;
+  };
+  for (size_t j = 0; j + 1 < J; ++j) {
+    CHK(copy(F, ck + j * n));
+    CHK(propagate_segment<T>(h, s, 1, F, B, C, j * C));
+    CHK(copy(ck + (j + 1) * n, F + C * n));
+  }
+  for (size_t j = J; j-- > 0;) {
+    const size_t s0 = j * C, s1 = std::min(s0 + C, Ns), c = s1 - s0;
+    const bool last = j + 1 == J;
+    CHK(copy(F, ck + j * n));
+    if (last) {
+      hipLaunchKernelGGL((crys::fill_kernel<T>), dim3(g), dim3(256), 0, s, B + c * n, (T)1, n);
+      HIPCHK(hipGetLastError());
+    } else {
+      CHK(copy(B + c * n, B));  // q_b at slot s1, left in slot 0 by the segment after
+    }
+    CHK(propagate_segment<T>(h, s, 2, F, B, c, s0));
+    hipLaunchKernelGGL((crys::phi_accumulate_kernel<T>), dim3(crys::red_blocks(n)),
+                       dim3(crys::kRedThreads), 0, s, (const T*)F, (const T*)B, n, s0,
+                       last ? c + 1 : c, h->NsA, Ns, h->phiA, h->phiB, last ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    if (stress) CHK(scft_record_stress<T>(h, s, F, B, s0, s1, s0));
+  }
+  return CRYS_OK;
 }
 
 // Records one full SCFT iteration (crysSCFT_sample.cpp:476-508) on stream s.
 // The forward chain is q_A then q_B (joined at slot NsA, :487), the backward
 // chain q_B^dagger then q_A^dagger (joined at slot NsA, :490); both advance
-// in the same launches.
+// in the same launches.  With stress, the per-step stress partials of the
+// same propagation are recorded too.
 template <typename T>
-int scft_record_iteration(crys_scft* h, hipStream_t s, double acc_mix) {
-  const crys_solver* sv = h->sv;
-  const OpSet& op = sv->ops[sv->cur];
-  const size_t n = h->n, Ns = h->Ns, NsA = h->NsA;
+int scft_record_iteration(crys_scft* h, hipStream_t s, double acc_mix, bool stress = false) {
+  const size_t n = h->n, Ns = h->Ns;
   T* qf = (T*)h->qf;
   T* qb = (T*)h->qb;
-  T* scr[2] = {(T*)h->scr[0], (T*)h->scr[1]};
-  const T* EA = (const T*)h->EA;
-  const T* EB = (const T*)h->EB;
-  if (phase_mp(sv)) {
-    for (size_t k = 0; k <= Ns; ++k)
-      CHK(launch_phase_step<T>(sv, op, k, Ns, NsA, qf, qb, scr, EA, EB, s));
+  if (h->every) {
+    CHK(record_checkpointed<T>(h, s, stress));
   } else {
-    for (size_t t = 0; t < Ns; ++t) {
-      const size_t bo = Ns - t - 1;
-      const T* in[2] = {qf + t * n, qb + (Ns - t) * n};
-      T* out[2] = {qf + (t + 1) * n, qb + bo * n};
-      const T* E[2] = {t < NsA ? EA : EB, bo >= NsA ? EB : EA};
-      CHK(launch_step<T>(sv, op, 2, in, out, scr, E, s));
-    }
+    CHK(propagate_segment<T>(h, s, 2, qf, qb, Ns, 0));
+    if (stress) CHK(scft_record_stress<T>(h, s, qf, qb, 0, Ns, 0));
   }
   const int nb = crys::red_blocks(n);
   hipLaunchKernelGGL((crys::qsum_partial_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
@@ -777,10 +888,16 @@ int scft_record_iteration(crys_scft* h, hipStream_t s, double acc_mix) {
   hipLaunchKernelGGL(crys::q_finalize_kernel, dim3(1), dim3(crys::kRedThreads), 0, s,
                      (const double*)h->qpart, nb, h->scal, n, h->fA, h->fB, h->NsA, h->NsB);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL((crys::phi_energy_mix_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
-                     (const T*)qf, (const T*)qb, (const double*)h->scal, h->phiA, h->phiB, h->wA,
-                     h->wB, h->ksi, (T*)h->EA, (T*)h->EB, h->epart, n, h->NsA, h->NsB, h->xN,
-                     acc_mix, h->ds);
+  if (h->every)
+    hipLaunchKernelGGL((crys::phi_energy_mix_kernel<T, true>), dim3(nb), dim3(crys::kRedThreads),
+                       0, s, (const T*)nullptr, (const T*)nullptr, (const double*)h->scal,
+                       h->phiA, h->phiB, h->wA, h->wB, h->ksi, (T*)h->EA, (T*)h->EB, h->epart, n,
+                       h->NsA, h->NsB, h->xN, acc_mix, h->ds);
+  else
+    hipLaunchKernelGGL((crys::phi_energy_mix_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
+                       (const T*)qf, (const T*)qb, (const double*)h->scal, h->phiA, h->phiB,
+                       h->wA, h->wB, h->ksi, (T*)h->EA, (T*)h->EB, h->epart, n, h->NsA, h->NsB,
+                       h->xN, acc_mix, h->ds);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(crys::energy_finalize_kernel, dim3(1), dim3(crys::kRedThreads), 0, s,
                      (const double*)h->epart, nb, (const double*)h->scal, h->trace, h->counter,
@@ -789,40 +906,78 @@ int scft_record_iteration(crys_scft* h, hipStream_t s, double acc_mix) {
   return CRYS_OK;
 }
 
-// d Q / d L_a summed over the contour: for every step s and axis a,
-// < E_s q^dagger_{s+1}, (dD_a (x) D_b (x) D_c) E_s q_s > with the derivative
-// operator on axis a.  Partials go to h->spart[(3 s + a) * nb + block].
-template <typename T>
-int scft_record_stress(crys_scft* h, hipStream_t s) {
-  const crys_solver* sv = h->sv;
-  const OpSet& op = sv->ops[sv->cur];
-  const size_t n = h->n, Ns = h->Ns, NsA = h->NsA;
-  const T* qf = (const T*)h->qf;
-  const T* qb = (const T*)h->qb;
-  T* w = (T*)h->scr[0];
-  const int nb = crys::red_blocks(n);
-  crys::PassParams<T> p{};
-  p.X = (int)sv->nphy[0];
-  p.Y = (int)sv->nphy[1];
-  p.Z = (int)sv->nphy[2];
-  for (size_t t = 0; t < Ns; ++t) {
-    const T* E = (const T*)(t < NsA ? h->EA : h->EB);
-    for (int a = 0; a < 3; ++a) {
-      // z pass (pre E), then y, then x; the axis a pass uses dD/dL_a
-      const int order[3] = {2, 1, 0};
-      for (int q = 0; q < 3; ++q) {
-        const int ax = order[q];
-        p.c[0] = {q == 0 ? qf + t * n : w, w, q == 0 ? E : nullptr, nullptr};
-        p.mat = (const T*)(ax == a ? op.dmat_dL[ax] : op.dmat[ax]);
-        p.mat_lo = ax == a ? nullptr : (const unsigned short*)op.dmat_lo[ax];
-        HIPCHK(crys::launch_axis_pass<T>(ax, p, 1, s));
-      }
-      hipLaunchKernelGGL((crys::dot3_partial_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0, s,
-                         (const T*)w, E, qb + (t + 1) * n, h->spart + (3 * t + a) * nb, n);
-      HIPCHK(hipGetLastError());
-    }
+// Bytes of contour storage for checkpoint interval `every` (0: full histories).
+size_t contour_bytes(size_t n, size_t Ns, size_t every, size_t es) {
+  if (every == 0) return 2 * (Ns + 1) * n * es;
+  const size_t nck = (Ns + every - 1) / every;
+  return (nck + 2 * (every + 1)) * n * es;
+}
+
+// Allocates the contour storage on first use: full histories when they fit
+// the budget (the requested one, else 90% of the free device memory at this
+// call), else checkpoints every ceil(sqrt(Ns)) slots; a requested interval
+// is used as given.
+int scft_storage(crys_scft* h) {
+  if (h->qf) return CRYS_OK;
+  const size_t n = h->n, Ns = h->Ns, es = elem_size(h->sv->precision);
+  size_t budget = h->budget;
+  if (budget == 0) {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    budget = fr / 10 * 9;
   }
+  size_t every = h->every_req;
+  if (every == 0 && contour_bytes(n, Ns, 0, es) > budget)
+    every = (size_t)std::ceil(std::sqrt((double)Ns));
+  if (every >= Ns) every = 0;  // one segment: the full histories
+  const size_t bytes = contour_bytes(n, Ns, every, es);
+  if (h->budget && bytes > h->budget)
+    return fail(CRYS_ERR_OUT_OF_MEMORY,
+                "contour storage needs %zu bytes (checkpoint interval %zu), budget %zu", bytes,
+                every, h->budget);
+  const size_t slots = every ? every + 1 : Ns + 1;
+  const size_t nck = every ? (Ns + every - 1) / every : 0;
+  hipError_t e = hipMalloc(&h->qf, slots * n * es);
+  if (e == hipSuccess) e = hipMalloc(&h->qb, slots * n * es);
+  if (e == hipSuccess && nck) e = hipMalloc(&h->ck, nck * n * es);
+  if (e != hipSuccess) {
+    for (void** p : {&h->qf, &h->qb, &h->ck}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+    return fail(e == hipErrorOutOfMemory ? CRYS_ERR_OUT_OF_MEMORY : CRYS_ERR_HIP,
+                "contour storage (%zu bytes): %s", bytes, hipGetErrorString(e));
+  }
+  h->every = every;
+  h->nck = nck;
+  // the start values: q_f[0] = 1 and q_b[Ns] = 1 (crysSCFT_sample.cpp:483, :486);
+  // with checkpoints, q_f[0] is checkpoint 0 and q_b[Ns] is set per iteration
+  hipStream_t s = h->sv->stream;
+  const unsigned g = grid_for(n);
+  if (es == sizeof(float)) {
+    hipLaunchKernelGGL((crys::fill_kernel<float>), dim3(g), dim3(256), 0, s,
+                       (float*)(every ? h->ck : h->qf), 1.0f, n);
+    if (!every)
+      hipLaunchKernelGGL((crys::fill_kernel<float>), dim3(g), dim3(256), 0, s,
+                         (float*)h->qb + Ns * n, 1.0f, n);
+  } else {
+    hipLaunchKernelGGL((crys::fill_kernel<double>), dim3(g), dim3(256), 0, s,
+                       (double*)(every ? h->ck : h->qf), 1.0, n);
+    if (!every)
+      hipLaunchKernelGGL((crys::fill_kernel<double>), dim3(g), dim3(256), 0, s,
+                         (double*)h->qb + Ns * n, 1.0, n);
+  }
+  HIPCHK(hipGetLastError());
   return CRYS_OK;
+}
+
+void scft_free_storage(crys_scft* h) {
+  for (void** p : {&h->qf, &h->qb, &h->ck}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  h->every = 0;
+  h->nck = 0;
 }
 
 int scft_build_graph(crys_scft* h) {
@@ -831,6 +986,7 @@ int scft_build_graph(crys_scft* h) {
   if (h->graph) (void)hipGraphDestroy(h->graph);
   h->exec = nullptr;
   h->graph = nullptr;
+  CHK(scft_storage(h));
   HIPCHK(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
   int r = sv->precision == CRYS_FP32 ? scft_record_iteration<float>(h, sv->stream, h->acc)
                                      : scft_record_iteration<double>(h, sv->stream, h->acc);
@@ -853,15 +1009,6 @@ void scft_prep(crys_scft* h) {
   hipLaunchKernelGGL((crys::prep_fields_kernel<T>), dim3(grid_for(h->n)), dim3(256), 0,
                      h->sv->stream, (const double*)h->wA, (const double*)h->wB, h->ksi, (T*)h->EA,
                      (T*)h->EB, h->xN, h->ds, h->n);
-}
-
-template <typename T>
-void scft_fill_ends(crys_scft* h) {
-  const size_t n = h->n;
-  hipLaunchKernelGGL((crys::fill_kernel<T>), dim3(grid_for(n)), dim3(256), 0, h->sv->stream,
-                     (T*)h->qf, (T)1, n);
-  hipLaunchKernelGGL((crys::fill_kernel<T>), dim3(grid_for(n)), dim3(256), 0, h->sv->stream,
-                     (T*)h->qb + h->Ns * n, (T)1, n);
 }
 
 }  // namespace
@@ -919,8 +1066,6 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
   A((void**)&h->phiB, n * 8);
   A(&h->EA, n * es);
   A(&h->EB, n * es);
-  A(&h->qf, (h->Ns + 1) * n * es);
-  A(&h->qb, (h->Ns + 1) * n * es);
   // two fields per chain: the phases alternate between them (a phase may
   // run on two workgroups per plane, so it cannot rewrite its input in place)
   A(&h->scr[0], 2 * n * es);
@@ -936,10 +1081,7 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
     return fail(e == hipErrorOutOfMemory ? CRYS_ERR_OUT_OF_MEMORY : CRYS_ERR_HIP,
                 "SCFT allocation failed: %s", hipGetErrorString(e));
   }
-  if (solver->precision == CRYS_FP32) scft_fill_ends<float>(h);
-  else scft_fill_ends<double>(h);
-  e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(h->sv->stream);
+  e = hipStreamSynchronize(h->sv->stream);
   if (e != hipSuccess) {
     scft_free(h);
     delete h;
@@ -954,6 +1096,41 @@ int crys_scft_destroy(crys_scft_t* h) {
   if (h->sv && h->sv->stream) (void)hipStreamSynchronize(h->sv->stream);
   scft_free(h);
   delete h;
+  return CRYS_OK;
+}
+
+int crys_scft_set_checkpointing(crys_scft_t* h, size_t every, size_t budget_bytes) {
+  if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
+  HIPCHK(hipStreamSynchronize(h->sv->stream));
+  // the iteration graph records the storage addresses: rebuild it on next use
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->exec = nullptr;
+  h->graph = nullptr;
+  scft_free_storage(h);
+  const size_t old_every = h->every_req, old_budget = h->budget;
+  h->every_req = every;
+  h->budget = budget_bytes;
+  const int r = scft_storage(h);
+  if (r != CRYS_OK) {
+    // keep the handle usable with the previous setting
+    std::string keep = g_err;
+    h->every_req = old_every;
+    h->budget = old_budget;
+    (void)scft_storage(h);
+    g_err = keep;
+    return r;
+  }
+  HIPCHK(hipStreamSynchronize(h->sv->stream));
+  return CRYS_OK;
+}
+
+int crys_scft_get_checkpointing(crys_scft_t* h, size_t* every, size_t* storage_bytes) {
+  if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
+  CHK(scft_storage(h));
+  if (every) *every = h->every;
+  if (storage_bytes)
+    *storage_bytes = contour_bytes(h->n, h->Ns, h->every, elem_size(h->sv->precision));
   return CRYS_OK;
 }
 
@@ -1077,6 +1254,7 @@ int crys_scft_evaluate(crys_scft_t* h, const double* wA, const double* wB, doubl
   const size_t n = h->n, Ns = h->Ns;
   const int nb = crys::red_blocks(n);
   const bool stress = dH_dL != nullptr;
+  CHK(scft_storage(h));
   if (!h->gA) HIPCHK(hipMalloc((void**)&h->gA, n * 8));
   if (!h->gB) HIPCHK(hipMalloc((void**)&h->gB, n * 8));
   if (!h->sums) HIPCHK(hipMalloc((void**)&h->sums, (1 + 3 * Ns) * 8));
@@ -1087,7 +1265,8 @@ int crys_scft_evaluate(crys_scft_t* h, const double* wA, const double* wB, doubl
   hipLaunchKernelGGL(crys::set_int_kernel, dim3(1), dim3(1), 0, s, h->counter, 0);
   HIPCHK(hipGetLastError());
   const bool f32 = sv->precision == CRYS_FP32;
-  CHK(f32 ? scft_record_iteration<float>(h, s, 0.0) : scft_record_iteration<double>(h, s, 0.0));
+  CHK(f32 ? scft_record_iteration<float>(h, s, 0.0, stress)
+          : scft_record_iteration<double>(h, s, 0.0, stress));
   const double cA = h->ds * (double)h->NsA / h->fA, cB = h->ds * (double)h->NsB / h->fB;
   hipLaunchKernelGGL(crys::hgrad_kernel, dim3(nb), dim3(crys::kRedThreads), 0, s,
                      (const double*)h->wA, (const double*)h->wB, (const double*)h->phiA,
@@ -1097,7 +1276,6 @@ int crys_scft_evaluate(crys_scft_t* h, const double* wA, const double* wB, doubl
                      (const double*)h->qpart, nb, h->sums);
   HIPCHK(hipGetLastError());
   if (stress) {
-    CHK(f32 ? scft_record_stress<float>(h, s) : scft_record_stress<double>(h, s));
     hipLaunchKernelGGL(crys::sum_partials_kernel, dim3((unsigned)(3 * Ns)), dim3(crys::kRedThreads),
                        0, s, (const double*)h->spart, nb, h->sums + 1);
     HIPCHK(hipGetLastError());
@@ -1166,7 +1344,9 @@ int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* m
   if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
   const OpSet& op = sv->ops[sv->cur];
   hipStream_t s = sv->stream;
-  const size_t n = h->n, Ns = h->Ns;
+  CHK(scft_storage(h));
+  // the slots held in qf / qb: the whole contour, or one checkpoint segment
+  const size_t n = h->n, Ns = h->every ? h->every : h->Ns;
   int nl = 0;
   const bool phase = phase_mp(sv) != 0 && Ns >= 2;
   // one contour step of both chains (q_0 -> q_1 forward, q+_Ns -> q+_Ns-1

@@ -24,6 +24,13 @@ __global__ void fill_kernel(T* p, T v, size_t n) {
     p[i] = v;
 }
 
+template <typename T>
+__global__ void copy_kernel(T* dst, const T* src, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 template <typename D, typename S>
 __global__ void convert_kernel(D* dst, const S* src, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -102,12 +109,36 @@ static __global__ __launch_bounds__(kRedThreads) void q_finalize_kernel(const do
   }
 }
 
+// integrate_phi's contour sums (:348-360) over the slots s0 .. s0+cnt-1 of
+// one checkpoint segment: qf and qb hold those slots contiguously.  The
+// trapezoid weights are those of the whole contour (slot 0, NsA and Ns weigh
+// 1/2 in the blocks that end there).  first != 0 starts the sums, else they
+// are added to accA / accB; every slot is summed in exactly one segment.
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void phi_accumulate_kernel(
+    const T* qf, const T* qb, size_t n, size_t s0, size_t cnt, size_t NsA, size_t Ns,
+    double* accA, double* accB, int first) {
+  for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kRedThreads) {
+    double a = 0.0, b = 0.0;
+    for (size_t k = 0; k < cnt; ++k) {
+      const size_t t = s0 + k;
+      const double p = (double)qf[k * n + i] * (double)qb[k * n + i];
+      if (t <= NsA) a += (t == 0 || t == NsA) ? 0.5 * p : p;
+      if (t >= NsA) b += (t == NsA || t == Ns) ? 0.5 * p : p;
+    }
+    accA[i] = first ? a : accA[i] + a;
+    accB[i] = first ? b : accB[i] + b;
+  }
+}
+
 // integrate_phi (:348-360) for both blocks from the joined histories
 // (forward slots 0..NsA are q_A, NsA..NsA+NsB are q_B; backward slots the
 // same for q^dagger), then print_energy's sums (:404-418) with the
 // pre-update fields, calc_new_fields (:362-371), and the next iteration's
-// calc_ksi / array_exp (:478-480).
-template <typename T>
+// calc_ksi / array_exp (:478-480).  With FROM_SUMS, phiA / phiB hold the
+// contour sums of phi_accumulate_kernel on entry and qf / qb are not read.
+template <typename T, bool FROM_SUMS = false>
 __global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
     const T* qf, const T* qb, const double* scal, double* phiA, double* phiB, double* wA,
     double* wB, double* ksi, T* EA, T* EB, double* part, size_t n, size_t NsA, size_t NsB,
@@ -139,8 +170,8 @@ __global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
                                  (double)qf[s1 * n + i] * (double)qb[s1 * n + i]);
       return (p0 + p1) + (p2 + p3) + ends;
     };
-    const double pA = ca * contour_sum(0, NsA);
-    const double pB = cb * contour_sum(NsA, NsA + NsB);
+    const double pA = ca * (FROM_SUMS ? phiA[i] : contour_sum(0, NsA));
+    const double pB = cb * (FROM_SUMS ? phiB[i] : contour_sum(NsA, NsA + NsB));
     const double a = wA[i], b = wB[i], x = ksi[i];
     s_e1 += -a * pA - b * pB;
     s_enth += xN * pA * pB;

@@ -68,6 +68,20 @@ class SCFT:
         except Exception:
             pass
 
+    def set_checkpointing(self, every: int = 0, budget_bytes: int = 0):
+        """Contour checkpointing (crys_scft_set_checkpointing): keep the
+        forward chain every ``every`` slots and recompute between them; with
+        every=0, full histories when they fit ``budget_bytes`` (0: free device
+        memory), else every = ceil(sqrt(Ns))."""
+        check(lib().crys_scft_set_checkpointing(self._h, int(every), int(budget_bytes)))
+
+    def checkpointing(self):
+        """(interval in use, 0 for full histories; contour storage in bytes)."""
+        e = ctypes.c_size_t()
+        b = ctypes.c_size_t()
+        check(lib().crys_scft_get_checkpointing(self._h, ctypes.byref(e), ctypes.byref(b)))
+        return e.value, b.value
+
     def set_fields(self, wA, wB):
         a = np.ascontiguousarray(wA, dtype=np.float64).reshape(-1)
         b = np.ascontiguousarray(wB, dtype=np.float64).reshape(-1)

@@ -156,11 +156,27 @@ enum crys_trace_col {
 /* Sets up an AB-diblock run on `solver` (not owned): NsA = (size_t)(fA/ds),
  * NsB = (size_t)((1-fA)/ds) (crysSCFT_sample.cpp:443-444), calls
  * set_cell_para({Lx,Ly,Lz}) and set_contour_step(ds) (:466-467) and the
- * warm-up diffusion (:471), and allocates the device fields and the
- * (NsA+NsB+1)-slot forward and backward propagator histories. */
+ * warm-up diffusion (:471), and allocates the device fields.  The contour
+ * storage (the (NsA+NsB+1)-slot forward and backward propagator histories
+ * of :447-460, or checkpoints, see crys_scft_set_checkpointing) is allocated
+ * on first use. */
 CRYS_API int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, double Lx, double Ly,
                      double Lz, double acceptance, crys_scft_t** out);
 CRYS_API int crys_scft_destroy(crys_scft_t* h);
+/* Contour checkpointing.  No counterpart in the reference, whose driver keeps
+ * every q(s) of the four chains (crysSCFT_sample.cpp:447-460).  every > 0
+ * keeps the forward chain only at the slots 0, every, 2 every, ... and
+ * recomputes the segments between them, so that an iteration (and
+ * crys_scft_evaluate, stress included) holds ceil(Ns/every) + 2 (every + 1)
+ * fields instead of 2 (Ns + 1), for one more propagation of the forward
+ * chain.  every == 0 picks the interval: full histories when they fit in
+ * budget_bytes (0: 90% of the free device memory when the storage is
+ * allocated), else every = ceil(sqrt(Ns)).  Fails with
+ * CRYS_ERR_OUT_OF_MEMORY when a nonzero budget cannot hold the storage.
+ * Results equal the full-history ones to rounding. */
+CRYS_API int crys_scft_set_checkpointing(crys_scft_t* h, size_t every, size_t budget_bytes);
+/* The interval in use (0: full histories) and the contour storage in bytes. */
+CRYS_API int crys_scft_get_checkpointing(crys_scft_t* h, size_t* every, size_t* storage_bytes);
 /* Initial fields in the solver's physical layout (init_func, :463). */
 CRYS_API int crys_scft_set_fields(crys_scft_t* h, const double* wA_host, const double* wB_host);
 /* Runs nstep iterations of :476-508 on the device.  trace_host (may be NULL)
