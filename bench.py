@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -80,6 +81,57 @@ def run_gpu(precision, steps, warmup, wA, wB):
     run.close()
     solver.close()
     return out
+
+
+def checkpoint_timing(steps):
+    """One JSON line: the headline workload with full histories and with
+    checkpoints every ceil(sqrt(Ns)) slots (crys_scft_set_checkpointing):
+    iterations/s, contour storage and the free-energy difference, FP64 and
+    FP32; plus one gradient-and-stress evaluation of each."""
+    import crysscft_sample_amd as C
+    from crysscft_sample_amd import fields as F
+
+    wfA, wfB = F.bcc_fields((N_LOG,) * 3, L, FA, XN)
+    wA, wB = F.extract(wfA, "PmmmDCT"), F.extract(wfB, "PmmmDCT")
+    out = {"metric": "SCFT iter/s at 128^3 Ns=200 PmmmDCT, full histories vs sqrt(Ns) "
+                     "checkpoints", "unit": "iter/s", "higher_is_better": True, "steps": steps}
+    for prec in ("fp64", "fp32"):
+        res = {}
+        for every in (0, None):
+            s = C.SolverPmmmDCT((N_LOG,) * 3, precision=prec)
+            run = C.SCFT(s, FA, DS, XN, L, ACC)
+            Ns = run.NsA + run.NsB
+            run.set_checkpointing(every if every is not None else math.ceil(math.sqrt(Ns)), 0)
+            used, nbytes = run.checkpointing()
+            run.set_fields(wA, wB)
+            run.run(2)
+            C.lib().crys_device_synchronize()
+            t0 = time.perf_counter()
+            tr = run.run(steps)
+            C.lib().crys_device_synchronize()
+            dt = time.perf_counter() - t0
+            t1 = time.perf_counter()
+            g = run.evaluate(wA, wB, stress=True)
+            te = time.perf_counter() - t1
+            res[used] = dict(iter_s=steps / dt, bytes=nbytes, trace=tr, eval_ms=te * 1e3, H=g["H"],
+                             dL=g["dH_dL"])
+            run.close()
+            s.close()
+        full, (every, ck) = res[0], [(k, v) for k, v in res.items() if k][0]
+        out[prec] = {"every": every,
+                     "full_iter_s": round(full["iter_s"], 3),
+                     "checkpointed_iter_s": round(ck["iter_s"], 3),
+                     "full_storage_mib": round(full["bytes"] / 2**20, 1),
+                     "checkpointed_storage_mib": round(ck["bytes"] / 2**20, 1),
+                     "max_rel_dF": float(np.abs(ck["trace"][:, 0] / full["trace"][:, 0] - 1).max()),
+                     "evaluate_ms_full": round(full["eval_ms"], 2),
+                     "evaluate_ms_checkpointed": round(ck["eval_ms"], 2),
+                     "evaluate_rel_dH": abs(ck["H"] / full["H"] - 1),
+                     "evaluate_rel_dstress": float(np.abs(ck["dL"] - full["dL"]).max()
+                                                   / np.abs(full["dL"]).max())}
+        print(f"bench: checkpoint {prec} done", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def cpu_baseline(wA, wB, threads):
@@ -205,6 +257,8 @@ def main():
                     help="run the fixed workload of the rocprofv3 --pmc passes and exit")
     ap.add_argument("--triplet", action="store_true",
                     help="time single diffusion() triplets (the reference's clock metric) and exit")
+    ap.add_argument("--checkpoint", action="store_true",
+                    help="time full histories against sqrt(Ns) checkpoints and exit")
     ap.add_argument("--batch", type=int, default=0, metavar="COUNT",
                     help="time COUNT batched instances (the batching workload) and exit")
     args = ap.parse_args()
@@ -216,6 +270,8 @@ def main():
         return batch_timing(args.batch, max(args.steps // 10, 3))
     if args.pmc_workload:
         return pmc_workload()
+    if args.checkpoint:
+        return checkpoint_timing(max(args.steps // 2, 5))
     if args.triplet:
         return triplet_timing(200)
 

@@ -407,32 +407,27 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
     acc[u] = V4{T(0), T(0), T(0), T(0)};
     if (active) {
       const int col = (sub * TPW + u) * 16 + (lane & 15);
+      T b[KS];
 #pragma unroll
-      for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
-        T b[4];
+      for (int n = 0; n < KS; ++n) b[n] = tile[brow[n] * LDT + col];
+      // low parts first (see slab_product)
+      if constexpr (SPLIT) {
+        if (use_lo) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int n = 4 * c4 + i;
-          if (n < KS) {
-            b[i] = tile[brow[n] * LDT + col];
-            acc[u] = MfmaOp<T>::mma(a[n], b[i], acc[u]);
-          } else {
-            b[i] = T(0);
-          }
-        }
-        if constexpr (SPLIT) {
-          if (use_lo) {
+          for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
             typename MfmaOp<float>::s4 av, bv;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
               const int n = 4 * c4 + i;
               av[i] = n < KS ? alo[n] : (short)0;
-              bv[i] = f32_to_bf16(b[i]);
+              bv[i] = n < KS ? f32_to_bf16(b[n]) : (short)0;
             }
             acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
           }
         }
       }
+#pragma unroll
+      for (int n = 0; n < KS; ++n) acc[u] = MfmaOp<T>::mma(a[n], b[n], acc[u]);
     }
   }
   __syncthreads();
@@ -607,32 +602,28 @@ __device__ __forceinline__ void slab_product(const SlabOperands<T, MP, CEN>& op,
     const int ct = S::col_tile(w, u);
     if (ct < 0) continue;
     const int col = ct * 16 + (lane & 15);
+    T b[KS];
 #pragma unroll
-    for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
-      T b[4];
+    for (int n = 0; n < KS; ++n) b[n] = lds[op.brow[n] * P + col];
+    // the operator's low parts first, while the accumulator is small (added
+    // last, a correction below half an ulp of the sum would be rounded away)
+    if constexpr (SPLIT) {
+      if (use_lo) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int n = 4 * c4 + i;
-        if (n < KS) {
-          b[i] = lds[op.brow[n] * P + col];
-          acc[u] = MfmaOp<T>::mma(op.a[n], b[i], acc[u]);
-        } else {
-          b[i] = T(0);
-        }
-      }
-      if constexpr (SPLIT) {
-        if (use_lo) {
+        for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
           typename MfmaOp<float>::s4 av, bv;
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int n = 4 * c4 + i;
             av[i] = n < KS ? op.alo[n] : (short)0;
-            bv[i] = f32_to_bf16(b[i]);
+            bv[i] = n < KS ? f32_to_bf16(b[n]) : (short)0;
           }
           acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
         }
       }
     }
+#pragma unroll
+    for (int n = 0; n < KS; ++n) acc[u] = MfmaOp<T>::mma(op.a[n], b[n], acc[u]);
   }
 }
 

@@ -98,19 +98,12 @@ __device__ __forceinline__ void phase_row_product(
 #pragma unroll
     for (int n = 0; n < KS; ++n) b[u][n] = on[u] ? lds[op.brow[n] * PB + col] : T(0);
   }
+  // the operator's low parts first, while the accumulators are small (added
+  // last, a correction below half an ulp of the sum would be rounded away)
+  if constexpr (SPLIT) {
+    if (use_lo) {
 #pragma unroll
-  for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int n = 4 * c4 + i;
-      if (n < KS) {
-#pragma unroll
-        for (int u = 0; u < TPW; ++u)
-          if (on[u]) acc[u] = MfmaOp<T>::mma(op.a[n], b[u][n], acc[u]);
-      }
-    }
-    if constexpr (SPLIT) {
-      if (use_lo) {
+      for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
 #pragma unroll
         for (int u = 0; u < TPW; ++u) {
           if (!on[u]) continue;
@@ -125,6 +118,12 @@ __device__ __forceinline__ void phase_row_product(
         }
       }
     }
+  }
+#pragma unroll
+  for (int n = 0; n < KS; ++n) {
+#pragma unroll
+    for (int u = 0; u < TPW; ++u)
+      if (on[u]) acc[u] = MfmaOp<T>::mma(op.a[n], b[u][n], acc[u]);
   }
 }
 
@@ -150,19 +149,11 @@ __device__ __forceinline__ void phase_col_product(
 #pragma unroll
     for (int n = 0; n < KS; ++n) a[u][n] = on[u] ? lds[row * PA + op.brow[n]] : T(0);
   }
+  // low parts first (see phase_row_product)
+  if constexpr (SPLIT) {
+    if (use_lo) {
 #pragma unroll
-  for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int n = 4 * c4 + i;
-      if (n < KS) {
-#pragma unroll
-        for (int u = 0; u < TPW; ++u)
-          if (on[u]) acc[u] = MfmaOp<T>::mma(a[u][n], op.a[n], acc[u]);
-      }
-    }
-    if constexpr (SPLIT) {
-      if (use_lo) {
+      for (int c4 = 0; c4 < (KS + 3) / 4; ++c4) {
 #pragma unroll
         for (int u = 0; u < TPW; ++u) {
           if (!on[u]) continue;
@@ -177,6 +168,12 @@ __device__ __forceinline__ void phase_col_product(
         }
       }
     }
+  }
+#pragma unroll
+  for (int n = 0; n < KS; ++n) {
+#pragma unroll
+    for (int u = 0; u < TPW; ++u)
+      if (on[u]) acc[u] = MfmaOp<T>::mma(a[u][n], op.a[n], acc[u]);
   }
 }
 
@@ -420,12 +417,11 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_b(const SlabOperands
 #pragma unroll
   for (int n = 0; n < KS; ++n) b[n] = lds[op.brow[n] * pitch + col];
   V4 acc = V4{T(0), T(0), T(0), T(0)};
+  // low parts first (see phase_row_product)
+  if constexpr (SPLIT) {
+    if (use_lo) {
 #pragma unroll
-  for (int c4 = 0; c4 < KS / 4; ++c4) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(op.a[4 * c4 + i], b[4 * c4 + i], acc);
-    if constexpr (SPLIT) {
-      if (use_lo) {
+      for (int c4 = 0; c4 < KS / 4; ++c4) {
         typename MfmaOp<float>::s4 av, bv;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -436,6 +432,8 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_b(const SlabOperands
       }
     }
   }
+#pragma unroll
+  for (int n = 0; n < KS; ++n) acc = MfmaOp<T>::mma(op.a[n], b[n], acc);
   return acc;
 }
 
@@ -452,12 +450,11 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a(const SlabOperands
 #pragma unroll
   for (int n = 0; n < KS; ++n) a[n] = lds[row * pitch + op.brow[n]];
   V4 acc = V4{T(0), T(0), T(0), T(0)};
+  // low parts first (see phase_row_product)
+  if constexpr (SPLIT) {
+    if (use_lo) {
 #pragma unroll
-  for (int c4 = 0; c4 < KS / 4; ++c4) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(a[4 * c4 + i], op.a[4 * c4 + i], acc);
-    if constexpr (SPLIT) {
-      if (use_lo) {
+      for (int c4 = 0; c4 < KS / 4; ++c4) {
         typename MfmaOp<float>::s4 av, bv;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -468,6 +465,8 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a(const SlabOperands
       }
     }
   }
+#pragma unroll
+  for (int n = 0; n < KS; ++n) acc = MfmaOp<T>::mma(a[n], op.a[n], acc);
   return acc;
 }
 
@@ -513,8 +512,7 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_b_khalf(
 #pragma unroll
   for (int i = 0; i < 4; ++i) b[i] = lds[op.brow[i] * pitch + col];
   V4 acc = V4{T(0), T(0), T(0), T(0)};
-#pragma unroll
-  for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(op.a[i], b[i], acc);
+  // low parts first (see phase_row_product)
   if constexpr (std::is_same<T, float>::value) {
     if (use_lo) {
       typename MfmaOp<float>::s4 av, bv;
@@ -526,6 +524,8 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_b_khalf(
       acc = MfmaOp<float>::mma_bf16(av, bv, acc);
     }
   }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(op.a[i], b[i], acc);
   return acc;
 }
 
@@ -538,8 +538,7 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a_khalf(
 #pragma unroll
   for (int i = 0; i < 4; ++i) a[i] = lds[row * pitch + op.brow[i]];
   V4 acc = V4{T(0), T(0), T(0), T(0)};
-#pragma unroll
-  for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(a[i], op.a[i], acc);
+  // low parts first (see phase_row_product)
   if constexpr (std::is_same<T, float>::value) {
     if (use_lo) {
       typename MfmaOp<float>::s4 av, bv;
@@ -551,6 +550,8 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a_khalf(
       acc = MfmaOp<float>::mma_bf16(av, bv, acc);
     }
   }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc = MfmaOp<T>::mma(a[i], op.a[i], acc);
   return acc;
 }
 
