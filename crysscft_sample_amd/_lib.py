@@ -103,6 +103,9 @@ SIGNATURES = {
     "crys_batch_create": (ctypes.c_int, [_c_vp, _c_sz, ctypes.c_int, ctypes.POINTER(_c_vp)]),
     "crys_batch_destroy": (ctypes.c_int, [_c_vp]),
     "crys_batch_set_fields": (ctypes.c_int, [_c_vp, _c_sz, _c_dp, _c_dp]),
+    "crys_batch_set_path": (ctypes.c_int, [_c_vp, ctypes.c_int]),
+    "crys_batch_get_path": (ctypes.c_int, [_c_vp, _c_ip]),
+    "crys_batch_autotune": (ctypes.c_int, [_c_vp, ctypes.c_int, _c_ip, _c_dp]),
     "crys_batch_run": (ctypes.c_int, [_c_vp, _c_sz, _c_dp]),
     "crys_batch_get_fields": (ctypes.c_int, [_c_vp, _c_sz, _c_dp, _c_dp, _c_dp, _c_dp]),
     "crys_batch_get_info": (ctypes.c_int, [_c_vp, _c_sz, ctypes.POINTER(_c_sz),

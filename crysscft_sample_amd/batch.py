@@ -90,6 +90,25 @@ class Batch:
         check(lib().crys_batch_run(self._h, int(nstep), dptr(out)))
         return out
 
+    PATHS = {"auto": 0, "phases": 1, "passes": 2}
+
+    def set_path(self, path: str):
+        """'auto', 'phases' (one launch per contour step) or 'passes' (three)."""
+        check(lib().crys_batch_set_path(self._h, self.PATHS[path]))
+
+    def path(self) -> str:
+        p = ctypes.c_int()
+        check(lib().crys_batch_get_path(self._h, ctypes.byref(p)))
+        return {1: "phases", 2: "passes"}[p.value]
+
+    def autotune(self, reps: int = 3):
+        """Times the contour steps on each possible path, keeps the faster.
+        Returns (path, ms of the phases or None, ms of the passes)."""
+        p = ctypes.c_int()
+        ms = (ctypes.c_double * 2)()
+        check(lib().crys_batch_autotune(self._h, int(reps), ctypes.byref(p), ms))
+        return ({1: "phases", 2: "passes"}[p.value], ms[0] if ms[0] >= 0 else None, ms[1])
+
     def fields(self, k):
         sh = self.shape(k)
         phiA, phiB, wA, wB = (np.empty(sh) for _ in range(4))

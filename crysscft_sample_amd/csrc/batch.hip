@@ -326,6 +326,7 @@ struct crys_batch {
   std::vector<int> cls;                           // 3 per instance
   std::vector<size_t> off_m;                      // matrix offsets, 3 per instance
   std::vector<size_t> off_mc;                     // half-size matrix offsets (cen instances)
+  bool phase_ok = false;                          // every instance can run the phases
   bool phases = false;                            // one launch per step (batch_phase_kernel)
   int2* pwork[2] = {nullptr, nullptr};            // phase items: x-slab, y-slab phases
   std::vector<int> pprefix[2];                    // phase items active at phase k
@@ -404,8 +405,8 @@ int batch_setup(crys_batch* b) {
     const size_t red = d.kind == CRYS_P1 ? 1 : 2;
     items += 2 * std::max(d.grid_log[0], d.grid_log[1]) / red;
   }
-  all = all && items <= phase_batch_items();
-  b->phases = all;
+  b->phase_ok = all;
+  b->phases = all && items <= phase_batch_items();
   if (!all) std::fill(cen.begin(), cen.end(), 0);
   b->off_mc.assign(3 * C, 0);
   for (size_t i = 0; i < C; ++i) {
@@ -427,7 +428,7 @@ int batch_setup(crys_batch* b) {
   BCHK(balloc(b, &b->qf, tot_q * sizeof(T)));
   BCHK(balloc(b, &b->qb, tot_q * sizeof(T)));
   // one scratch field per chain; the phases alternate between two
-  BCHK(balloc(b, &b->scr, (b->phases ? 4 : 2) * tot_f * sizeof(T)));
+  BCHK(balloc(b, &b->scr, (b->phase_ok ? 4 : 2) * tot_f * sizeof(T)));
   BCHK(balloc(b, &b->EA, tot_f * sizeof(T)));
   BCHK(balloc(b, &b->EB, tot_f * sizeof(T)));
   BCHK(balloc(b, &b->mats, tot_m * sizeof(T)));
@@ -526,8 +527,8 @@ int batch_setup(crys_batch* b) {
     I.qb = qb + b->off_q[i];
     I.scr0 = scr + b->off_f[i];
     I.scr1 = scr + (b->off_f[i] + tot_f);
-    I.scr0b = b->phases ? scr + (b->off_f[i] + 2 * tot_f) : nullptr;
-    I.scr1b = b->phases ? scr + (b->off_f[i] + 3 * tot_f) : nullptr;
+    I.scr0b = b->phase_ok ? scr + (b->off_f[i] + 2 * tot_f) : nullptr;
+    I.scr1b = b->phase_ok ? scr + (b->off_f[i] + 3 * tot_f) : nullptr;
     I.EA = (T*)b->EA + b->off_f[i];
     I.EB = (T*)b->EB + b->off_f[i];
     I.wA = b->wA + b->off_f[i];
@@ -573,7 +574,7 @@ int batch_setup(crys_batch* b) {
     BCHK(balloc(b, (void**)&b->work[a], wl.size() * sizeof(int2)));
     BHIP(hipMemcpy(b->work[a], wl.data(), wl.size() * sizeof(int2), hipMemcpyHostToDevice));
   }
-  if (b->phases) {
+  if (b->phase_ok) {
     for (int ys = 0; ys < 2; ++ys) {
       std::vector<int2> wl;
       std::vector<int> end_of;
@@ -616,10 +617,12 @@ int batch_setup(crys_batch* b) {
   return CRYS_OK;
 }
 
+// The contour steps of one iteration of every instance: the phases (one
+// launch per step) or the per-axis passes (three).
 template <typename T>
-int batch_record(crys_batch* b, hipStream_t s) {
+int batch_record_steps(crys_batch* b, hipStream_t s, bool phases) {
   const crys::BatchInst<T>* I = (const crys::BatchInst<T>*)b->insts;
-  for (int k = 0; b->phases && k <= b->Ns_max; ++k) {
+  for (int k = 0; phases && k <= b->Ns_max; ++k) {
     const int ys = k % 2;
     const int cnt = b->pprefix[ys][k];
     if (cnt == 0) continue;
@@ -627,7 +630,7 @@ int batch_record(crys_batch* b, hipStream_t s) {
                        (const int2*)b->pwork[ys], k, ys);
     BHIP(hipGetLastError());
   }
-  for (int t = 0; !b->phases && t < b->Ns_max; ++t) {
+  for (int t = 0; !phases && t < b->Ns_max; ++t) {
     const int ax_order[3] = {2, 1, 0};
     for (int a : ax_order) {
       const int cnt = b->prefix[a][t];
@@ -644,6 +647,13 @@ int batch_record(crys_batch* b, hipStream_t s) {
       BHIP(hipGetLastError());
     }
   }
+  return CRYS_OK;
+}
+
+template <typename T>
+int batch_record(crys_batch* b, hipStream_t s) {
+  const crys::BatchInst<T>* I = (const crys::BatchInst<T>*)b->insts;
+  BCHK(batch_record_steps<T>(b, s, b->phases));
   const unsigned C = (unsigned)b->count;
   hipLaunchKernelGGL((crys::batch_qsum_kernel<T>), dim3(b->n_rwork), dim3(crys::kRedThreads), 0,
                      s, I, (const int2*)b->rwork, b->part_q);
@@ -657,6 +667,40 @@ int batch_record(crys_batch* b, hipStream_t s) {
   hipLaunchKernelGGL(crys::batch_count_kernel, dim3(1), dim3(1), 0, s, b->counter, 0);
   BHIP(hipGetLastError());
   return CRYS_OK;
+}
+
+void batch_drop_graph(crys_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Mean milliseconds of the contour steps of one iteration on one path, over
+// reps back-to-back recordings after one warm-up.  The steps only write the
+// histories and the scratch fields, which the next iteration rewrites, so
+// timing them leaves the batch's state as it was.
+int batch_time_steps(crys_batch* b, bool phases, int reps, double* ms) {
+  hipEvent_t e0, e1;
+  BHIP(hipEventCreate(&e0));
+  BHIP(hipEventCreate(&e1));
+  auto rec = [&]() {
+    return b->precision == CRYS_FP32 ? batch_record_steps<float>(b, b->stream, phases)
+                                     : batch_record_steps<double>(b, b->stream, phases);
+  };
+  int r = rec();
+  if (r == CRYS_OK) r = hipEventRecord(e0, b->stream) == hipSuccess ? CRYS_OK : CRYS_ERR_HIP;
+  for (int i = 0; i < reps && r == CRYS_OK; ++i) r = rec();
+  if (r == CRYS_OK) r = hipEventRecord(e1, b->stream) == hipSuccess ? CRYS_OK : CRYS_ERR_HIP;
+  float t = 0.f;
+  if (r == CRYS_OK && hipEventSynchronize(e1) == hipSuccess &&
+      hipEventElapsedTime(&t, e0, e1) == hipSuccess)
+    *ms = (double)t / reps;
+  else if (r == CRYS_OK)
+    r = errf(CRYS_ERR_HIP, "event timing failed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return r;
 }
 
 int batch_graph(crys_batch* b) {
@@ -809,6 +853,49 @@ int crys_batch_run(crys_batch_t* b, size_t nstep, double* trace) {
     done += chunk;
   }
   if (diverged) return errf(CRYS_ERR_DIVERGED, "non-finite free energy or Q in the trace");
+  return CRYS_OK;
+}
+
+int crys_batch_set_path(crys_batch_t* b, int path) {
+  if (!b) return errf(CRYS_ERR_INVALID_ARGUMENT, "null handle");
+  if (path < CRYS_BATCH_PATH_AUTO || path > CRYS_BATCH_PATH_PASSES)
+    return errf(CRYS_ERR_INVALID_ARGUMENT, "unknown path %d", path);
+  if (path == CRYS_BATCH_PATH_PHASES && !b->phase_ok)
+    return errf(CRYS_ERR_UNSUPPORTED,
+                "the phases need every instance's three axes in one length class");
+  bool ph = path == CRYS_BATCH_PATH_PHASES;
+  if (path == CRYS_BATCH_PATH_AUTO) {
+    size_t items = 0;
+    for (const crys_instance_t& d : b->desc)
+      items += 2 * std::max(d.grid_log[0], d.grid_log[1]) / (d.kind == CRYS_P1 ? 1 : 2);
+    ph = b->phase_ok && items <= phase_batch_items();
+  }
+  if (ph != b->phases) {
+    BHIP(hipStreamSynchronize(b->stream));
+    batch_drop_graph(b);
+    b->phases = ph;
+  }
+  return CRYS_OK;
+}
+
+int crys_batch_get_path(const crys_batch_t* b, int* path) {
+  if (!b || !path) return errf(CRYS_ERR_INVALID_ARGUMENT, "null argument");
+  *path = b->phases ? CRYS_BATCH_PATH_PHASES : CRYS_BATCH_PATH_PASSES;
+  return CRYS_OK;
+}
+
+int crys_batch_autotune(crys_batch_t* b, int reps, int* path, double ms[2]) {
+  if (!b || reps <= 0) return errf(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
+  double t_ph = -1.0, t_pa = -1.0;
+  if (b->phase_ok) BCHK(batch_time_steps(b, true, reps, &t_ph));
+  BCHK(batch_time_steps(b, false, reps, &t_pa));
+  const bool ph = b->phase_ok && t_ph < t_pa;
+  BCHK(crys_batch_set_path(b, ph ? CRYS_BATCH_PATH_PHASES : CRYS_BATCH_PATH_PASSES));
+  if (path) *path = ph ? CRYS_BATCH_PATH_PHASES : CRYS_BATCH_PATH_PASSES;
+  if (ms) {
+    ms[0] = t_ph;
+    ms[1] = t_pa;
+  }
   return CRYS_OK;
 }
 

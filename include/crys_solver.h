@@ -228,6 +228,29 @@ CRYS_API int crys_batch_set_fields(crys_batch_t* b, size_t i, const double* wA, 
 CRYS_API int crys_batch_run(crys_batch_t* b, size_t nstep, double* trace);
 CRYS_API int crys_batch_get_fields(crys_batch_t* b, size_t i, double* phiA, double* phiB,
                                    double* wA, double* wB);
+
+/* How a batch advances its contour steps.  No counterpart in the reference,
+ * which runs one instance at a time.  PHASES: one launch per step for every
+ * instance (possible when each instance has its three axes in one length
+ * class); PASSES: three per-axis launches per step; AUTO: the phases while the
+ * batch holds at most 6144 (instance, chain, plane) items, else the passes. */
+enum crys_batch_path {
+  CRYS_BATCH_PATH_AUTO = 0,
+  CRYS_BATCH_PATH_PHASES = 1,
+  CRYS_BATCH_PATH_PASSES = 2
+};
+/* Selects the path (CRYS_ERR_UNSUPPORTED for PHASES on a batch that cannot
+ * run them).  Results agree to rounding on both paths. */
+CRYS_API int crys_batch_set_path(crys_batch_t* b, int path);
+/* The path in use: CRYS_BATCH_PATH_PHASES or CRYS_BATCH_PATH_PASSES. */
+CRYS_API int crys_batch_get_path(const crys_batch_t* b, int* path);
+/* Times the contour steps of one iteration on each possible path (reps
+ * repetitions after a warm-up, HIP events on the batch's stream), selects the
+ * faster and returns it in *path; ms (may be NULL) receives the mean
+ * milliseconds of the phases (-1 when not possible) and of the passes.  The
+ * timed steps write only the histories and scratch fields, so the batch's
+ * fields and trace are unchanged. */
+CRYS_API int crys_batch_autotune(crys_batch_t* b, int reps, int* path, double ms[2]);
 CRYS_API int crys_batch_get_info(const crys_batch_t* b, size_t i, size_t* n_phy, size_t* NsA,
                                  size_t* NsB);
 

@@ -125,3 +125,62 @@ def test_phase_batch_matches_oracle_per_instance(oracle, prec):
             assert dF <= 1e-10 and dM <= 1e-10
         else:
             assert dF < 1e-5
+
+
+def _phase_batch_run(prec, path, nstep, autotune=False):
+    b = Batch(PHASE_BATCH, precision=prec)
+    for k, inst in enumerate(PHASE_BATCH):
+        b.set_fields(k, *seeded_fields(inst, 400 + k))
+    tuned = None
+    if path:
+        b.set_path(path)
+    tr = b.run(nstep)
+    if autotune:
+        tuned = b.autotune(reps=2)
+        tr = np.concatenate([tr, b.run(1)], axis=1)
+    used = b.path()
+    phiA = b.fields(1)[0]
+    b.close()
+    return tr, phiA, used, tuned
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_paths_agree(prec):
+    """The one-launch phases and the per-axis passes give the same traces."""
+    ph, phi_ph, used_ph, _ = _phase_batch_run(prec, "phases", 3)
+    pa, phi_pa, used_pa, _ = _phase_batch_run(prec, "passes", 3)
+    assert (used_ph, used_pa) == ("phases", "passes")
+    d = rel_err(ph, pa)
+    say(f"  {prec} phases vs passes: trace {d:.2e}, phiA {rel_err(phi_ph, phi_pa):.2e}")
+    assert d < (1e-10 if prec == "fp64" else 1e-6)
+
+
+def test_batch_autotune_keeps_state():
+    """Autotune times both paths, keeps the faster and leaves the fields as they
+    were: 3 iterations, autotune, 1 iteration equals 4 iterations on that path."""
+    tr, _, used, (path, ms_ph, ms_pa) = _phase_batch_run("fp64", None, 3, autotune=True)
+    say(f"  autotune: phases {ms_ph:.3f} ms, passes {ms_pa:.3f} ms -> {path}")
+    assert path == used and ms_ph > 0 and ms_pa > 0
+    assert path == ("phases" if ms_ph < ms_pa else "passes")
+    ref, _, _, _ = _phase_batch_run("fp64", path, 4)
+    assert rel_err(tr[:, 3], ref[:, 3]) < 1e-10
+    assert rel_err(tr[:, :3], ref[:, :3]) < 1e-10
+
+
+def test_batch_path_rules():
+    """A batch with mixed length classes runs only the passes."""
+    b = Batch(MIXED[:4], precision="fp64")
+    assert b.path() == "passes"
+    with pytest.raises(C.CrysError) as e:
+        b.set_path("phases")
+    assert e.value.status == C._lib.ERR_UNSUPPORTED
+    path, ms_ph, ms_pa = b.autotune(reps=1)
+    assert (path, ms_ph) == ("passes", None) and ms_pa > 0
+    b.close()
+    small = Batch(PHASE_BATCH, precision="fp64")
+    assert small.path() == "phases"  # automatic: a small batch of one-class instances
+    small.set_path("passes")
+    assert small.path() == "passes"
+    small.set_path("auto")
+    assert small.path() == "phases"
+    small.close()
