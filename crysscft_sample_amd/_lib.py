@@ -94,6 +94,7 @@ SIGNATURES = {
         _c_dp, _c_dp, _c_dp]),
     "crys_axis_operator": (ctypes.c_int, [ctypes.c_int, _c_sz, ctypes.c_double, ctypes.c_double,
                                           _c_dp]),
+    "crys_scft_time_propagation": (ctypes.c_int, [_c_vp, ctypes.c_int, _c_dp, _c_ip]),
     "crys_scft_time_step": (ctypes.c_int, [_c_vp, ctypes.c_int, _c_dp, _c_dp, _c_ip]),
     "crys_output_fields": (ctypes.c_int, [_c_dp, _c_dp, _c_dp, _c_dp, _c_sz, ctypes.c_char_p,
                                           ctypes.c_int]),

@@ -1337,6 +1337,51 @@ int crys_axis_operator(int kind, size_t n_logical, double L, double ds, double* 
   return CRYS_OK;
 }
 
+int crys_scft_time_propagation(crys_scft_t* h, int reps, double* ms_per_step,
+                                int* launches_per_step) {
+  if (!h || reps <= 0) return fail(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
+  crys_solver* sv = h->sv;
+  if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
+  CHK(scft_storage(h));
+  hipStream_t s = sv->stream;
+  // the propagation of both chains exactly as the iteration records it (one
+  // segment of `every` steps when checkpointing), captured in a graph
+  const size_t c = h->every ? h->every : h->Ns;
+  const bool f32 = sv->precision == CRYS_FP32;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int r = f32 ? propagate_segment<float>(h, s, 2, (float*)h->qf, (float*)h->qb, c, 0)
+                    : propagate_segment<double>(h, s, 2, (double*)h->qf, (double*)h->qb, c, 0);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (r != CRYS_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return fail(CRYS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  size_t nodes = 0;
+  hipGraphExec_t x = nullptr;
+  e = hipGraphGetNodes(g, nullptr, &nodes);
+  if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipGraphLaunch(x, s);  // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int i = 0; i < reps && e == hipSuccess; ++i) e = hipGraphLaunch(x, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (x) (void)hipGraphExecDestroy(x);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(CRYS_ERR_HIP, "propagation timing: %s", hipGetErrorString(e));
+  if (ms_per_step) *ms_per_step = (double)ms / ((double)reps * (double)c);
+  if (launches_per_step) *launches_per_step = (int)((nodes + c / 2) / c);
+  return CRYS_OK;
+}
+
 int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* ms_per_launch,
                         int* nkernels) {
   if (!h || reps <= 0) return fail(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");

@@ -121,6 +121,15 @@ class SCFT:
                                        dptr(gB), None if dL is None else dptr(dL), dptr(row)))
         return {"H": H.value, "dH_dwA": gA, "dH_dwB": gB, "dH_dL": dL, "trace_row": row}
 
+    def time_propagation(self, reps: int = 5):
+        """(ms per contour step of both chains, launches per step) over the
+        iteration's own launch sequence, captured in a hipGraph."""
+        a = ctypes.c_double()
+        nl = ctypes.c_int()
+        check(lib().crys_scft_time_propagation(self._h, int(reps), ctypes.byref(a),
+                                               ctypes.byref(nl)))
+        return a.value, nl.value
+
     def time_step(self, reps: int = 50):
         """(ms per contour step of both chains, [ms per launch of each kernel
         of the step, in issue order]), timed with HIP events on the stream."""

@@ -318,6 +318,14 @@ CRYS_API int crys_axis_operator(int kind, size_t n_logical, double L, double ds,
  * the z and y passes are fused on x-slabs, else 3).  Any output may be null.
  * No counterpart in the reference, which times diffusion() with std::clock
  * (crysSCFT_sample.cpp:389-391). */
+/* Times the propagation of both chains as one SCFT iteration runs it: the
+ * Ns-step (or, with checkpointing, one segment's) launch sequence on the
+ * handle's own buffers, captured in a hipGraph and launched `reps` times
+ * between HIP events after a warm-up.  Writes the mean milliseconds per
+ * contour step (graph time / (reps * steps)) and the number of launches per
+ * step (rounded).  Any output may be null.  No counterpart in the reference. */
+CRYS_API int crys_scft_time_propagation(crys_scft_t* h, int reps, double* ms_per_step,
+                                        int* launches_per_step);
 CRYS_API int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step,
                                  double* ms_per_launch, int* nkernels);
 
