@@ -36,6 +36,13 @@ def golden():
     }
 
 
+@pytest.fixture(scope="session")
+def golden_scft():
+    """The oracle's 200-iteration SCFT traces of the 64^3 workload
+    (tests/golden/make_scft_traces.py)."""
+    return np.load(os.path.join(GOLDEN, "scft_cfg1.npz"))
+
+
 def rel_err(a, b):
     a = np.asarray(a)
     b = np.asarray(b)

@@ -293,3 +293,33 @@ def test_scft_one_launch_phases(oracle, name, N, cell, prec):
     else:
         assert dF < 1e-5
         assert np.abs(gphiA - phiA).max() < 1e-4
+
+
+@pytest.mark.parametrize("name", ["PmmmDCT", "3m", "P1"])
+def test_scft_200_iterations_against_golden(golden_scft, name):
+    """The GPU workload in full: 64^3 grid, Ns = 200, 200 mixing iterations,
+    every iteration's F and MaxIncompressibility within 1e-10 of the oracle's
+    trace (tests/golden/scft_cfg1.npz, make_scft_traces.py), and the final
+    phiA and wA at 64 fixed points."""
+    c = CFG1
+    N = c["N"]
+    wA, wB = F.bcc_fields((N,) * 3, c["L"], c["fA"], c["xN"])
+    a, b = F.extract(wA, name), F.extract(wB, name)
+    s = C.SOLVERS[name]((N,) * 3)
+    run = C.SCFT(s, c["fA"], c["ds"], c["xN"], (c["L"],) * 3, c["acc"])
+    assert (run.NsA, run.NsB) == (50, 150)
+    run.set_fields(a, b)
+    tr = run.run(c["nstep"])
+    phiA, _, wA_out, _ = run.fields()
+    ref = golden_scft[f"{name}_trace"]
+    idx = golden_scft[f"{name}_idx"]
+    dF = np.abs(tr[:, 0] / ref[:, 0] - 1).max()
+    dM = np.abs(tr[:, 5] / ref[:, 5] - 1).max()
+    say(f"  {name} 200 iterations: |dF/F| {dF:.2e} |dMaxInc| {dM:.2e}")
+    assert tr.shape == ref.shape == (200, 7)
+    assert dF <= 1e-10 and dM <= 1e-10
+    assert rel_err(tr, ref) < 1e-10
+    assert rel_err(phiA.reshape(-1)[idx], golden_scft[f"{name}_phiA"]) < 1e-10
+    assert rel_err(wA_out.reshape(-1)[idx], golden_scft[f"{name}_wA"]) < 1e-10
+    run.close()
+    s.close()

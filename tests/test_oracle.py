@@ -243,3 +243,25 @@ def test_3m_expansion_rejects_invalid_operations():
         F.expand_3m(red, [0.3, 0, 0, 0, 0, 0, 0, 0, 0])  # not whole grid points
     with pytest.raises(ValueError):
         F.expand_3m(red, [0, 0, 1 / 16, 0, 0, 0, 0, 0, 0])  # maps evens onto evens
+
+
+def test_scft_golden_traces_agree_across_solvers(oracle):
+    """tests/golden/scft_cfg1.npz (make_scft_traces.py): the three solvers'
+    200-iteration traces on the 64^3 GPU workload agree to 1e-10 in F and
+    MaxIncompressibility (the reference's acceptance criterion, Readme.md:28-76),
+    and the first iteration is what the oracle computes now."""
+    import os
+    from conftest import GOLDEN
+
+    g = np.load(os.path.join(GOLDEN, "scft_cfg1.npz"))
+    p = g["PmmmDCT_trace"]
+    assert p.shape == (200, 7)
+    for name in ("3m", "P1"):
+        t = g[f"{name}_trace"]
+        assert np.abs(t[:, 0] / p[:, 0] - 1).max() < 1e-10
+        assert np.abs(t[:, 5] / p[:, 5] - 1).max() < 1e-10
+    N, fA, xN, L, ds = 64, 0.25, 20.0, 4.0, 0.005
+    wA, wB = F.bcc_fields((N,) * 3, L, fA, xN)
+    s = oracle.Solver("PmmmDCT", (N,) * 3)
+    tr, *_ = s.scft(fA, ds, xN, L, 0.1, 1, F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT"))
+    assert np.abs(tr[0] - p[0]).max() <= 1e-13 * np.abs(p[0]).max()
