@@ -18,8 +18,12 @@
  *   - Driver: uniform-field closed form Q = exp(-ds(a NsA + b NsB)) with
  *     phiA == fA, phiB == fB; mean(phiA) = fA and mean(phiB) = fB for any
  *     field; d lnQ / dwA(r) = -(ds NsA / fA) phiA(r) / n by finite differences.
- *   - 3m with non-zero translational parts: parity unpinned (no fixture or
- *     closed form pins the glide-plane twiddles).
+ *   - 3m with non-zero translational parts: the phase-folded 3m equals P1
+ *     applied to the field expanded by the 8 symmetry operations, then
+ *     reduced to the even sub-lattice (6 translation sets, two of them with
+ *     shifted mirror planes that fix the direction of the translations).
+ *   - Memory safety: oracle/selftest.c under AddressSanitizer and
+ *     UndefinedBehaviorSanitizer.
  */
 #ifndef CRYS_ORACLE_H
 #define CRYS_ORACLE_H

@@ -323,3 +323,27 @@ def test_scft_200_iterations_against_golden(golden_scft, name):
     assert rel_err(wA_out.reshape(-1)[idx], golden_scft[f"{name}_wA"]) < 1e-10
     run.close()
     s.close()
+
+
+def test_non_finite_fields_report_divergence():
+    """A NaN in the fields makes the trace non-finite: crys_scft_run and
+    crys_scft_evaluate return CRYS_ERR_DIVERGED (the reference has no such
+    check and would print nan); the handle stays usable afterwards."""
+    N = (32, 32, 32)
+    wA, wB = F.bcc_fields(N, (4.0,) * 3, 0.3, 20.0)
+    a, b = F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT")
+    s = C.SolverPmmmDCT(N)
+    run = C.SCFT(s, 0.3, 0.01, 20.0, (4.0,) * 3, 0.1)
+    bad = a.copy()
+    bad[3, 4, 5] = np.nan
+    run.set_fields(bad, b)
+    with pytest.raises(C.CrysError) as e:
+        run.run(1)
+    assert e.value.status == _lib.ERR_DIVERGED
+    with pytest.raises(C.CrysError) as e:
+        run.evaluate(bad, b, stress=False)
+    assert e.value.status == _lib.ERR_DIVERGED
+    run.set_fields(a, b)
+    assert np.isfinite(run.run(2)).all()
+    run.close()
+    s.close()
