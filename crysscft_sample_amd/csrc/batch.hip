@@ -7,7 +7,7 @@
 // the whole batch is one launch of batch_phase_kernel: the alternating
 // x-slab / y-slab phases of plane_phase.hpp, with blockIdx.x indexing a work
 // list of (instance, chain, plane) items, provided that list holds at most
-// 6144 items (small and medium batches).  Otherwise a step is three launches
+// 1792 items (small batches).  Otherwise a step is three launches
 // of batch_pass_kernel, one per axis, over (instance, chain, tile) items.
 // Either list is ordered by decreasing contour length so that the instances
 // still running at step t are a prefix of it, and each item's length class
@@ -46,7 +46,7 @@ constexpr int kCap = 256;  // trace rows kept on the device between copies
 size_t phase_batch_items() {
   static const size_t lim = [] {
     const char* e = getenv("CRYS_BATCH_PHASE_ITEMS");
-    return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)6144;
+    return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)1792;
   }();
   return lim;
 }
@@ -118,6 +118,13 @@ __global__ __launch_bounds__(256) void batch_pass_kernel(const BatchInst<T>* __r
   if constexpr (AXIS == 2) ch = {q + s_in * n, scr, E, nullptr};
   else if constexpr (AXIS == 1) ch = {scr, scr, nullptr, nullptr};
   else ch = {scr, q + s_out * n, nullptr, E};
+  if (I.mat_c[AXIS]) {  // half-size product (the axis length is its class, 32 or 64)
+    const T* mc = I.mat_c[AXIS];
+    const unsigned short* lc = I.mat_c_lo[AXIS];
+    if (I.cls[AXIS] == 32) mfma_pass_tile<T, AXIS, 32, true>(ch, mc, lc, I.X, I.Y, I.Z, tidx, lds);
+    else mfma_pass_tile<T, AXIS, 64, true>(ch, mc, lc, I.X, I.Y, I.Z, tidx, lds);
+    return;
+  }
   const T* mat = I.mat[AXIS];
   const unsigned short* lo = I.mat_lo[AXIS];
   switch (I.cls[AXIS]) {
@@ -382,7 +389,9 @@ int batch_setup(crys_batch* b) {
   size_t tot_f = 0, tot_q = 0, tot_m = 0, tot_mc = 0;
   // the one-launch phases need every instance's three axes in one class;
   // centrosymmetric products where all three lengths equal a class of 32/64
-  std::vector<int> pcls(C, 0), cen(C, 0);
+  // per axis: the half-size product applies where the axis length equals its
+  // class of 32 or 64 (PmmmDCT and P1 operators are centrosymmetric)
+  std::vector<int> pcls(C, 0), cen(C, 0), cax(3 * C, 0);
   bool all = crys::phase_enabled();
   for (size_t i = 0; i < C; ++i) {
     const crys_instance_t& d = b->desc[i];
@@ -390,15 +399,21 @@ int batch_setup(crys_batch* b) {
     const int c = b->cls[3 * i];
     if (b->cls[3 * i + 1] == c && b->cls[3 * i + 2] == c) pcls[i] = c;
     else all = false;
-    bool ce = pcls[i] != 0 && c % 32 == 0 && d.kind != CRYS_3M && crys::centro_enabled();
-    for (int a = 0; a < 3; ++a) ce = ce && (int)(d.grid_log[a] / red) == c;
+    bool ce = pcls[i] != 0;
+    for (int a = 0; a < 3; ++a) {
+      const int ca = b->cls[3 * i + a];
+      cax[3 * i + a] = d.kind != CRYS_3M && crys::centro_enabled() && ca % 32 == 0 &&
+                       (int)(d.grid_log[a] / red) == ca;
+      ce = ce && cax[3 * i + a];
+    }
     cen[i] = ce ? 1 : 0;
   }
-  // Measured on MI355X (PmmmDCT instances of 16^3..64^3, FP64 / FP32
-  // phases over passes): 1.57 / 2.11 at 176 items, 1.52 / 1.71 at 656,
-  // 1.21 / 1.16 at 2240, 1.11 / 1.01 at 4752, 0.99 / 0.90 at 9296 and
-  // 0.93 / 0.92 at 18400, where the passes' lighter workgroups keep more
-  // waves in flight.
+  // Measured on MI355X (PmmmDCT instances of 16^3..64^3, contour steps of one
+  // iteration, passes over phases, FP64 / FP32): 1.34 / 1.35 at 656 items,
+  // 1.08 / 1.08 at 1264, 0.98 / 0.89 at 2240, 0.90 / 0.75 at 4752 and
+  // 0.81 / 0.70 at 18400: the passes' lighter workgroups (and their
+  // half-size products on every axis of length 32 or 64) win once the batch
+  // fills the chip.
   size_t items = 0;
   for (size_t i = 0; i < C; ++i) {
     const crys_instance_t& d = b->desc[i];
@@ -419,7 +434,7 @@ int batch_setup(crys_batch* b) {
       b->off_m[3 * i + a] = tot_m;
       const size_t M = d.kind == CRYS_P1 ? d.grid_log[a] : d.grid_log[a] / 2;
       tot_m += M * M;
-      if (cen[i]) {
+      if (cax[3 * i + a]) {
         b->off_mc[3 * i + a] = tot_mc;
         tot_mc += M * M / 2;
       }
@@ -469,13 +484,13 @@ int batch_setup(crys_batch* b) {
     BCHK(balloc(b, (void**)&b->mats_lo, tot_m * sizeof(unsigned short)));
     BHIP(hipMemcpy(b->mats_lo, lo.data(), tot_m * sizeof(unsigned short), hipMemcpyHostToDevice));
   }
-  if (tot_mc) {  // half-size forms (centro_half) of the cen instances' operators
+  if (tot_mc) {  // half-size forms (centro_half) of the centrosymmetric axes
     std::vector<T> mc(tot_mc);
     std::vector<unsigned short> mclo(std::is_same<T, float>::value ? tot_mc : 0);
     for (size_t i = 0; i < C; ++i) {
-      if (!cen[i]) continue;
       const crys_instance_t& d = b->desc[i];
       for (int a = 0; a < 3; ++a) {
+        if (!cax[3 * i + a]) continue;
         const int M = (int)(d.kind == CRYS_P1 ? d.grid_log[a] : d.grid_log[a] / 2);
         const size_t o = b->off_m[3 * i + a];
         std::vector<double> At(M * (size_t)M);
@@ -513,8 +528,9 @@ int batch_setup(crys_batch* b) {
       I.cls[a] = b->cls[3 * i + a];
       I.mat[a] = (const T*)b->mats + b->off_m[3 * i + a];
       I.mat_lo[a] = b->mats_lo ? b->mats_lo + b->off_m[3 * i + a] : nullptr;
-      I.mat_c[a] = cen[i] ? (const T*)b->mats_c + b->off_mc[3 * i + a] : nullptr;
-      I.mat_c_lo[a] = (cen[i] && b->mats_c_lo) ? b->mats_c_lo + b->off_mc[3 * i + a] : nullptr;
+      I.mat_c[a] = cax[3 * i + a] ? (const T*)b->mats_c + b->off_mc[3 * i + a] : nullptr;
+      I.mat_c_lo[a] =
+          (cax[3 * i + a] && b->mats_c_lo) ? b->mats_c_lo + b->off_mc[3 * i + a] : nullptr;
     }
     I.pcls = pcls[i];
     I.cen = cen[i];

@@ -233,7 +233,7 @@ CRYS_API int crys_batch_get_fields(crys_batch_t* b, size_t i, double* phiA, doub
  * which runs one instance at a time.  PHASES: one launch per step for every
  * instance (possible when each instance has its three axes in one length
  * class); PASSES: three per-axis launches per step; AUTO: the phases while the
- * batch holds at most 6144 (instance, chain, plane) items, else the passes. */
+ * batch holds at most 1792 (instance, chain, plane) items, else the passes. */
 enum crys_batch_path {
   CRYS_BATCH_PATH_AUTO = 0,
   CRYS_BATCH_PATH_PHASES = 1,
