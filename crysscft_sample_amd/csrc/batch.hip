@@ -139,11 +139,15 @@ __global__ __launch_bounds__(256) void batch_pass_kernel(const BatchInst<T>* __r
 // << 16 | plane); `yslab` selects y-slab phases (odd k).  Forward chain q_f[t]
 // -> q_f[t+1] with E_A for t < NsA; backward chain q_b[Ns-t] -> q_b[Ns-t-1]
 // with E_B while Ns-t-1 >= NsA; phase k writes scratch field k % 2.
-template <typename T>
-__global__ __launch_bounds__(512) void batch_phase_kernel(const BatchInst<T>* __restrict__ insts,
-                                                          const int2* __restrict__ work, int k,
-                                                          int yslab) {
-  __shared__ T lds[64 * PhaseShape<T, 64, false, 512>::PB];
+//
+// NT = 1024 threads hold every body but the dense class-64 one within 128
+// VGPRs (16 waves per CU); a batch with dense class-64 planes (3m, or P1
+// lengths 49..63) runs the NT = 512 kernel, which holds all bodies.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void batch_phase_kernel(const BatchInst<T>* __restrict__ insts,
+                                                         const int2* __restrict__ work, int k,
+                                                         int yslab) {
+  __shared__ T lds[64 * PhaseShape<T, 64, false, NT>::PB];
   const int2 w = work[blockIdx.x];
   const BatchInst<T>& I = insts[w.x];
   const int chain = w.y >> 16, plane = w.y & 0xffff;
@@ -171,17 +175,21 @@ __global__ __launch_bounds__(512) void batch_phase_kernel(const BatchInst<T>* __
   if (I.cen) {
     const T *mr = I.mat_c[ra], *mc = I.mat_c[2];
     const unsigned short *lr = I.mat_c_lo[ra], *lc = I.mat_c_lo[2];
-    if (I.pcls == 32) plane_phase_body<T, 32, true, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
-    else plane_phase_body<T, 64, true, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
+    if (I.pcls == 32) plane_phase_body<T, 32, true, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
+    else plane_phase_body<T, 64, true, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
     return;
   }
   const T *mr = I.mat[ra], *mc = I.mat[2];
   const unsigned short *lr = I.mat_lo[ra], *lc = I.mat_lo[2];
   switch (I.pcls) {
-    case 16: plane_phase_body<T, 16, false, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
-    case 32: plane_phase_body<T, 32, false, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
-    case 48: plane_phase_body<T, 48, false, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
-    default: plane_phase_body<T, 64, false, 512>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
+    case 16: plane_phase_body<T, 16, false, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
+    case 32: plane_phase_body<T, 32, false, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
+    case 48: plane_phase_body<T, 48, false, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds); break;
+    default:
+      // the host runs the NT = 512 kernel whenever a dense class-64 plane exists
+      if constexpr (NT == 512)
+        plane_phase_body<T, 64, false, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
+      break;
   }
 }
 
@@ -334,6 +342,7 @@ struct crys_batch {
   std::vector<size_t> off_m;                      // matrix offsets, 3 per instance
   std::vector<size_t> off_mc;                     // half-size matrix offsets (cen instances)
   bool phase_ok = false;                          // every instance can run the phases
+  bool phase_dense64 = false;                     // a dense class-64 plane: 512-thread phases
   bool phases = false;                            // one launch per step (batch_phase_kernel)
   int2* pwork[2] = {nullptr, nullptr};            // phase items: x-slab, y-slab phases
   std::vector<int> pprefix[2];                    // phase items active at phase k
@@ -421,6 +430,8 @@ int batch_setup(crys_batch* b) {
     items += 2 * std::max(d.grid_log[0], d.grid_log[1]) / red;
   }
   b->phase_ok = all;
+  b->phase_dense64 = false;
+  for (size_t i = 0; i < C; ++i) b->phase_dense64 = b->phase_dense64 || (pcls[i] == 64 && !cen[i]);
   b->phases = all && items <= phase_batch_items();
   if (!all) std::fill(cen.begin(), cen.end(), 0);
   b->off_mc.assign(3 * C, 0);
@@ -642,8 +653,12 @@ int batch_record_steps(crys_batch* b, hipStream_t s, bool phases) {
     const int ys = k % 2;
     const int cnt = b->pprefix[ys][k];
     if (cnt == 0) continue;
-    hipLaunchKernelGGL((crys::batch_phase_kernel<T>), dim3(cnt), dim3(512), 0, s, I,
-                       (const int2*)b->pwork[ys], k, ys);
+    if (b->phase_dense64)
+      hipLaunchKernelGGL((crys::batch_phase_kernel<T, 512>), dim3(cnt), dim3(512), 0, s, I,
+                         (const int2*)b->pwork[ys], k, ys);
+    else
+      hipLaunchKernelGGL((crys::batch_phase_kernel<T, 1024>), dim3(cnt), dim3(1024), 0, s, I,
+                         (const int2*)b->pwork[ys], k, ys);
     BHIP(hipGetLastError());
   }
   for (int t = 0; !phases && t < b->Ns_max; ++t) {

@@ -106,15 +106,24 @@ PHASE_BATCH = [
 ]
 
 
+# A dense class-64 plane (P1 of length 56) makes the batch run its 512-thread
+# phase kernel, which holds that body; without one it runs the 1024-thread one.
+PHASE_BATCH_DENSE64 = PHASE_BATCH[:4] + [
+    Instance("P1", (56, 56, 56), (4.3,) * 3, 0.29, 21.0, 1 / 110),          # 64, dense
+]
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_phase_batch_matches_oracle_per_instance(oracle, prec):
+@pytest.mark.parametrize("insts", [PHASE_BATCH, PHASE_BATCH_DENSE64], ids=["nt1024", "nt512"])
+def test_phase_batch_matches_oracle_per_instance(oracle, prec, insts):
     nstep = 3
-    b = Batch(PHASE_BATCH, precision=prec)
-    ws = [seeded_fields(inst, 400 + k) for k, inst in enumerate(PHASE_BATCH)]
+    b = Batch(insts, precision=prec)
+    assert b.path() == "phases"
+    ws = [seeded_fields(inst, 400 + k) for k, inst in enumerate(insts)]
     for k, (wA, wB) in enumerate(ws):
         b.set_fields(k, wA, wB)
     tr = b.run(nstep)
-    for k, inst in enumerate(PHASE_BATCH):
+    for k, inst in enumerate(insts):
         ref = oracle.Solver(inst.solver, inst.grid_log)
         t_ref, *_ = ref.scft(inst.fA, inst.ds, inst.xN, inst.cell, inst.acceptance, nstep, *ws[k])
         dF = np.abs(tr[k, :, 0] / t_ref[:, 0] - 1).max()
