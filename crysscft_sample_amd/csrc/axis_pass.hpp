@@ -260,14 +260,16 @@ __device__ __forceinline__ void far_first_order(float gc, int (&ord)[KS]) {
 
 template <int MP>
 struct MfmaShape {
+  static constexpr int NT = MP <= 64 ? 256 : 512;  // threads: one wave per row tile at 128
   static constexpr int TL = MP <= 32 ? 64 : 32;  // lines per workgroup
   static constexpr int LDT = TL + 16;            // LDS pitch: 4 B-rows hit distinct banks
   static constexpr int JT = MP / 16;             // 16-row output tiles
   static constexpr int TT = TL / 16;             // 16-line output tiles
   static constexpr int KS = MP / 4;              // MFMA k steps
-  static constexpr int WPJ = JT >= 4 ? 1 : 4 / JT;  // waves sharing one row tile
+  static constexpr int WPJ = (NT / 64) / JT >= 1 ? (NT / 64) / JT : 1;  // waves per row tile
   static constexpr int TPW = TT / WPJ;           // line tiles per wave
   static_assert(TT % WPJ == 0, "line tiles must split over the waves of a row tile");
+  static_assert(JT * WPJ <= NT / 64, "every row tile needs a wave");
 };
 
 // One workgroup's tile `tidx` of a pass on one chain; `tile` is LDS of at
@@ -282,7 +284,7 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
                                                int Y, int Z, int tidx, T* tile) {
   using S = MfmaShape<MP>;
   using V4 = typename MfmaOp<T>::v4;
-  constexpr int NT = 256;
+  constexpr int NT = S::NT;
   constexpr int TL = S::TL, LDT = S::LDT, JT = S::JT, WPJ = S::WPJ, TPW = S::TPW;
   constexpr int KS = CEN ? MP / 8 : MP / 4;
   constexpr int H = MP / 2;
@@ -489,7 +491,7 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
 }
 
 template <typename T, int AXIS, int MP, bool CEN = false>
-__global__ __launch_bounds__(256) void axis_pass_mfma_kernel(PassParams<T> p) {
+__global__ __launch_bounds__(MfmaShape<MP>::NT) void axis_pass_mfma_kernel(PassParams<T> p) {
   __shared__ T tile[MP * MfmaShape<MP>::LDT];
   mfma_pass_tile<T, AXIS, MP, CEN>(p.c[blockIdx.y], p.mat, p.mat_lo, p.X, p.Y, p.Z, blockIdx.x,
                                    tile);
@@ -853,7 +855,7 @@ template <typename T, int AXIS, int MP>
 hipError_t launch_axis_pass_mp(const PassParams<T>& p, int nchains, hipStream_t s) {
   const int M = AXIS == 0 ? p.X : (AXIS == 1 ? p.Y : p.Z);
   const long nlines = (long)p.X * p.Y * p.Z / M;
-  if constexpr (MP <= 64) {
+  if constexpr (MP <= 128) {
     if (p.cen || pass_impl() == PassImpl::Mfma) {
       using S = MfmaShape<MP>;
       const long nblk = (nlines + S::TL - 1) / S::TL;
@@ -861,13 +863,13 @@ hipError_t launch_axis_pass_mp(const PassParams<T>& p, int nchains, hipStream_t 
       if (p.cen) {
         if constexpr (MP % 32 == 0) {
           if (M != MP) return hipErrorInvalidValue;
-          hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP, true>), grid, dim3(256), 0, s,
+          hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP, true>), grid, dim3(S::NT), 0, s,
                              p);
           return hipGetLastError();
         }
         return hipErrorInvalidValue;
       }
-      hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP>), grid, dim3(256), 0, s, p);
+      hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP>), grid, dim3(S::NT), 0, s, p);
       return hipGetLastError();
     }
   }

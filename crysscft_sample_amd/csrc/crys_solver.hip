@@ -211,7 +211,7 @@ int upload_matrix(const crys_solver* h, const std::vector<double>& At, void** ds
 // For FP32 solvers `lo` (if not null) receives the bf16 bits of
 // D - (float)D, the low part of the split-precision product.  When `cen` is
 // not null it receives the half-size centrosymmetric forms (PmmmDCT and P1
-// axes of length M % 32 == 0; null elsewhere).
+// axes of length 32, 64 or 128; null elsewhere).
 int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3] = nullptr,
                     void* cen[3] = nullptr, void* cen_lo[3] = nullptr) {
   for (int a = 0; a < 3; ++a) {
@@ -221,7 +221,8 @@ int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3
     else if (h->kind == CRYS_P1) At = crys::p1_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
     else At = crys::m3_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
     CHK(upload_matrix(h, At, &dst[a], lo ? &lo[a] : nullptr));
-    if (cen && h->kind != CRYS_3M && M % 32 == 0 && M <= 64 && crys::centro_enabled()) {
+    if (cen && h->kind != CRYS_3M && M % 32 == 0 && M <= 128 && M != 96 &&
+        crys::centro_enabled()) {
       std::vector<double> C = crys::centro_half(At, M);
       CHK(upload_matrix(h, C, &cen[a], cen_lo ? &cen_lo[a] : nullptr));
     }

@@ -347,3 +347,47 @@ def test_non_finite_fields_report_divergence():
     assert np.isfinite(run.run(2)).all()
     run.close()
     s.close()
+
+
+# Axis length 128 (class 128: matrix-core passes of 512 threads, half-size
+# products where the length is exactly 128) next to 96 (dense class 128) and 64.
+LARGE = [("P1", (128, 96, 64), (4.4, 4.1, 3.9), 0.005),
+         ("PmmmDCT", (256, 192, 128), (5.0, 5.0, 5.0), 0.005),
+         ("3m", (256, 192, 128), (5.0, 4.6, 4.2), 0.005)]
+
+
+@pytest.mark.parametrize("name,N,cell,ds", LARGE)
+def test_diffusion_class_128(oracle, name, N, cell, ds):
+    say("diffusion class 128", name, N)
+    ref = oracle.Solver(name, N)
+    ref.set_cell_para(cell)
+    ref.set_contour_step(ds)
+    f = np.random.default_rng(7).standard_normal(ref.grid_size_phy)
+    want = ref.apply(f)
+    for prec, tol in (("fp64", 1e-13), ("fp32", 2e-5)):
+        gpu = C.SOLVERS[name](N, precision=prec)
+        gpu.set_cell_para(cell)
+        gpu.set_contour_step(ds)
+        err = rel_err(gpu.apply(f), want)
+        say(f"  {prec}: {err:.2e}")
+        assert err < tol
+        gpu.close()
+
+
+def test_scft_class_128(oracle):
+    """PmmmDCT 256^3 (128^3 reduced): the SCFT iteration on class-128 passes."""
+    N, cell, fA, xN, ds, acc, nstep = (256,) * 3, (5.0,) * 3, 0.3, 20.0, 0.05, 0.1, 2
+    wA, wB = F.bcc_fields(N, cell, fA, xN)
+    a, b = F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT")
+    tr_ref, _, _, phiA_ref, _ = oracle.Solver("PmmmDCT", N).scft(fA, ds, xN, cell, acc, nstep, a, b)
+    s = C.SolverPmmmDCT(N)
+    run = C.SCFT(s, fA, ds, xN, cell, acc)
+    run.set_fields(a, b)
+    tr = run.run(nstep)
+    dF = np.abs(tr[:, 0] / tr_ref[:, 0] - 1).max()
+    dM = np.abs(tr[:, 5] / tr_ref[:, 5] - 1).max()
+    say(f"  256^3 Pmmm: |dF/F| {dF:.2e} |dMaxInc| {dM:.2e}")
+    assert dF <= 1e-10 and dM <= 1e-10
+    assert rel_err(run.fields()[0], phiA_ref) < 1e-10
+    run.close()
+    s.close()
