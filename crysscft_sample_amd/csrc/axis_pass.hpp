@@ -509,7 +509,7 @@ inline int pass_class(int M) {
 }
 
 // Which product the passes use: matrix cores (default) or the FMA kernel.
-enum class PassImpl { Mfma, Valu };
+enum class PassImpl { Auto, Mfma, Valu, Radix };
 PassImpl pass_impl();
 
 // ---------------------------------------------------------------------------
@@ -856,7 +856,7 @@ hipError_t launch_axis_pass_mp(const PassParams<T>& p, int nchains, hipStream_t 
   const int M = AXIS == 0 ? p.X : (AXIS == 1 ? p.Y : p.Z);
   const long nlines = (long)p.X * p.Y * p.Z / M;
   if constexpr (MP <= 128) {
-    if (p.cen || pass_impl() == PassImpl::Mfma) {
+    if (p.cen || pass_impl() != PassImpl::Valu) {
       using S = MfmaShape<MP>;
       const long nblk = (nlines + S::TL - 1) / S::TL;
       dim3 grid((unsigned)nblk, (unsigned)nchains, 1);

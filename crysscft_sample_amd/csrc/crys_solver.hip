@@ -21,6 +21,7 @@
 #include "m3_glide.hpp"
 #include "operators.hpp"
 #include "plane_phase.hpp"
+#include "radix_pass.hpp"
 
 namespace {
 
@@ -102,12 +103,19 @@ bool slab_enabled() {
   return on;
 }
 
-// CRYS_AXIS_PASS=valu selects the FMA pass kernel (for A/B timing); the
-// matrix-core kernel is the default.
+// Per-axis pass kernel.  By default an axis of length 128 takes the radix
+// FFT/DCT pass (radix_pass.hpp), measured faster there than the matrix-core
+// product in FP64 and FP32, and every other axis the matrix-core pass (or the
+// fused z+y slab).  For A/B timing CRYS_AXIS_PASS=mfma forces the matrix-core
+// passes, CRYS_AXIS_PASS=radix the radix pass on every power-of-two axis and
+// CRYS_AXIS_PASS=valu the FMA pass kernel.
 PassImpl pass_impl() {
   static const PassImpl impl = [] {
     const char* e = getenv("CRYS_AXIS_PASS");
-    return (e && strcmp(e, "valu") == 0) ? PassImpl::Valu : PassImpl::Mfma;
+    if (e && strcmp(e, "valu") == 0) return PassImpl::Valu;
+    if (e && strcmp(e, "radix") == 0) return PassImpl::Radix;
+    if (e && strcmp(e, "mfma") == 0) return PassImpl::Mfma;
+    return PassImpl::Auto;
   }();
   return impl;
 }
@@ -121,6 +129,7 @@ struct OpSet {
   void* dmat_c[3];   // half-size centrosymmetric form (PmmmDCT/P1, M % 32 == 0) or null
   void* dmat_c_lo[3];
   void* dmat_dL[3];  // dD/dL per axis, built on first use (cell stress)
+  void* drad[3];     // radix-pass tables (radix_pass.hpp) when the radix passes are selected
 };
 
 struct crys_solver {
@@ -181,6 +190,7 @@ void free_ops(crys_solver* h) {
       if (o.dmat_c[a]) (void)hipFree(o.dmat_c[a]);
       if (o.dmat_c_lo[a]) (void)hipFree(o.dmat_c_lo[a]);
       if (o.dmat_dL[a]) (void)hipFree(o.dmat_dL[a]);
+      if (o.drad[a]) (void)hipFree(o.drad[a]);
     }
   h->ops.clear();
   h->cur = -1;
@@ -230,11 +240,35 @@ int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3
   return CRYS_OK;
 }
 
+// Radix-pass kind of the solver (radix_pass.hpp).
+int radix_kind(const crys_solver* h) {
+  return h->kind == CRYS_P1 ? crys::kRadixP1 : (h->kind == CRYS_PMMM_DCT ? crys::kRadixDct
+                                                                         : crys::kRadixM3);
+}
+
+// Whether axis `a` of this solver's pass-based contour steps runs the radix
+// pass (see pass_impl).
+bool radix_axis(const crys_solver* h, int a) {
+  const int M = (int)h->nphy[a];
+  switch (crys::pass_impl()) {
+    case crys::PassImpl::Radix: return crys::radix_length_ok(M);
+    case crys::PassImpl::Auto: return M == 128;
+    default: return false;
+  }
+}
+
 int build_ops(crys_solver* h, double ds, OpSet* out) {
   out->ds = ds;
   for (int a = 0; a < 3; ++a)
-    out->dmat[a] = out->dmat_lo[a] = out->dmat_c[a] = out->dmat_c_lo[a] = out->dmat_dL[a] = nullptr;
-  return upload_axis_ops(h, ds, false, out->dmat, out->dmat_lo, out->dmat_c, out->dmat_c_lo);
+    out->dmat[a] = out->dmat_lo[a] = out->dmat_c[a] = out->dmat_c_lo[a] = out->dmat_dL[a] =
+        out->drad[a] = nullptr;
+  CHK(upload_axis_ops(h, ds, false, out->dmat, out->dmat_lo, out->dmat_c, out->dmat_c_lo));
+  for (int a = 0; a < 3; ++a) {
+    if (!radix_axis(h, a)) continue;
+    const std::vector<double> t = crys::radix_table(radix_kind(h), (int)h->nphy[a], h->cell[a], ds);
+    CHK(upload_matrix(h, t, &out->drad[a], nullptr));
+  }
+  return CRYS_OK;
 }
 
 // One contour step E D E (or a bare D with E == nullptr) for 1 or 2 chains:
@@ -251,40 +285,57 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
   p.X = X;
   p.Y = Y;
   p.Z = Z;
-  // z and y fused on x-slabs when the two axes share a length class; the
-  // half-size centrosymmetric operators when both axes have them
-  crys::SlabParams<T> sp{};
-  sp.X = X;
-  sp.Y = Y;
-  sp.Z = Z;
-  const bool cen_zy = op.dmat_c[1] && op.dmat_c[2] && Y == Z;
-  for (int c = 0; c < nch; ++c) sp.c[c] = {in[c], scr[c], E[c], nullptr};
-  sp.mat_z = (const T*)(cen_zy ? op.dmat_c[2] : op.dmat[2]);
-  sp.mat_y = (const T*)(cen_zy ? op.dmat_c[1] : op.dmat[1]);
-  sp.lo_z = (const unsigned short*)(cen_zy ? op.dmat_c_lo[2] : op.dmat_lo[2]);
-  sp.lo_y = (const unsigned short*)(cen_zy ? op.dmat_c_lo[1] : op.dmat_lo[1]);
+  crys::RadixParams<T> rp{};
+  rp.X = X;
+  rp.Y = Y;
+  rp.Z = Z;
+  // one pass along `ax` with the chains' buffers `cc`: the radix pass when
+  // the axis has a radix table, else the matrix-core pass
+  auto pass = [&](int ax, const crys::PassChain<T>* cc) -> int {
+    if (op.drad[ax]) {
+      for (int c = 0; c < nch; ++c) rp.c[c] = cc[c];
+      rp.tab = (const T*)op.drad[ax];
+      HIPCHK(crys::launch_radix_pass<T>(ax, radix_kind(h), rp, nch, s));
+      return CRYS_OK;
+    }
+    for (int c = 0; c < nch; ++c) p.c[c] = cc[c];
+    p.cen = op.dmat_c[ax] != nullptr;
+    p.mat = (const T*)(p.cen ? op.dmat_c[ax] : op.dmat[ax]);
+    p.mat_lo = (const unsigned short*)(p.cen ? op.dmat_c_lo[ax] : op.dmat_lo[ax]);
+    HIPCHK(crys::launch_axis_pass<T>(ax, p, nch, s));
+    return CRYS_OK;
+  };
+  crys::PassChain<T> cc[2];
   int k = 0;
-  if (crys::slab_applies(Y, Z)) {
+  if (crys::slab_applies(Y, Z) && !op.drad[1] && !op.drad[2]) {
+    // z and y fused on x-slabs when the two axes share a length class; the
+    // half-size centrosymmetric operators when both axes have them
+    crys::SlabParams<T> sp{};
+    sp.X = X;
+    sp.Y = Y;
+    sp.Z = Z;
+    const bool cen_zy = op.dmat_c[1] && op.dmat_c[2] && Y == Z;
+    for (int c = 0; c < nch; ++c) sp.c[c] = {in[c], scr[c], E[c], nullptr};
+    sp.mat_z = (const T*)(cen_zy ? op.dmat_c[2] : op.dmat[2]);
+    sp.mat_y = (const T*)(cen_zy ? op.dmat_c[1] : op.dmat[1]);
+    sp.lo_z = (const unsigned short*)(cen_zy ? op.dmat_c_lo[2] : op.dmat_lo[2]);
+    sp.lo_y = (const unsigned short*)(cen_zy ? op.dmat_c_lo[1] : op.dmat_lo[1]);
     if (only < 0 || only == k) HIPCHK(crys::launch_slab_zy<T>(sp, nch, cen_zy, s));
     ++k;
   } else {
+    // z pass in -> scr (pre E), y pass in place on scr
     for (int ax = 2; ax >= 1; --ax, ++k) {
       if (only >= 0 && only != k) continue;
       for (int c = 0; c < nch; ++c)
-        p.c[c] = ax == 2 ? crys::PassChain<T>{in[c], scr[c], E[c], nullptr}
-                         : crys::PassChain<T>{scr[c], scr[c], nullptr, nullptr};
-      p.cen = op.dmat_c[ax] != nullptr;
-      p.mat = (const T*)(p.cen ? op.dmat_c[ax] : op.dmat[ax]);
-      p.mat_lo = (const unsigned short*)(p.cen ? op.dmat_c_lo[ax] : op.dmat_lo[ax]);
-      HIPCHK(crys::launch_axis_pass<T>(ax, p, nch, s));
+        cc[c] = ax == 2 ? crys::PassChain<T>{in[c], scr[c], E[c], nullptr}
+                        : crys::PassChain<T>{scr[c], scr[c], nullptr, nullptr};
+      CHK(pass(ax, cc));
     }
   }
+  // x pass scr -> out (post E)
   if (only < 0 || only == k) {
-    for (int c = 0; c < nch; ++c) p.c[c] = {scr[c], out[c], nullptr, E[c]};
-    p.cen = op.dmat_c[0] != nullptr;
-    p.mat = (const T*)(p.cen ? op.dmat_c[0] : op.dmat[0]);
-    p.mat_lo = (const unsigned short*)(p.cen ? op.dmat_c_lo[0] : op.dmat_lo[0]);
-    HIPCHK(crys::launch_axis_pass<T>(0, p, nch, s));
+    for (int c = 0; c < nch; ++c) cc[c] = {scr[c], out[c], nullptr, E[c]};
+    CHK(pass(0, cc));
   }
   ++k;
   if (nlaunch) *nlaunch = k;
@@ -568,6 +619,7 @@ int crys_set_contour_step(crys_solver_t* h, double ds) {
       if (o.dmat_lo[a]) (void)hipFree(o.dmat_lo[a]);
       if (o.dmat_c[a]) (void)hipFree(o.dmat_c[a]);
       if (o.dmat_c_lo[a]) (void)hipFree(o.dmat_c_lo[a]);
+      if (o.drad[a]) (void)hipFree(o.drad[a]);
     }
     return r;
   }

@@ -99,6 +99,24 @@ std::vector<double> centro_half(const std::vector<double>& At, int M) {
   return c;
 }
 
+std::vector<double> radix_table(int kind, int M, double L, double ds) {
+  std::vector<double> t(5 * (size_t)M);
+  for (int j = 0; j < M; ++j) {
+    // the angles 2 pi j / M and pi j / (2M) in long double
+    const long double a = 2.0L * kPiL * (long double)j / (long double)M;
+    const long double b = kPiL * (long double)j / (2.0L * (long double)M);
+    t[2 * j] = (double)cosl(a);
+    t[2 * j + 1] = (double)sinl(a);
+    t[2 * M + 2 * j] = (double)cosl(b);
+    t[2 * M + 2 * j + 1] = (double)sinl(b);
+    if (kind == 0)
+      t[4 * M + j] = (double)(kfactor(j < (M + 1) / 2 ? j : M - j, L, ds, false) / (long double)M);
+    else
+      t[4 * M + j] = (double)(kfactor(j, L, ds, false) / (2.0L * (long double)M));
+  }
+  return t;
+}
+
 unsigned short bf16_bits(float x) {
   unsigned u;
   std::memcpy(&u, &x, sizeof(u));

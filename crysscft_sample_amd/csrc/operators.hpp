@@ -38,6 +38,14 @@ std::vector<double> m3_axis_operator(int N_logical, double L, double ds, bool dL
 // At_c[k*M + M/2 + j] = Pa[j][k], Ps/Pa = (D[j][k] +/- D[j][M-1-k]) / 2.
 std::vector<double> centro_half(const std::vector<double>& At, int M);
 
+// Constant table of the radix pass (radix_pass.hpp) for one axis.  kind 0:
+// P1, transform length M = N (the logical length), K' = K(k folded) / N;
+// kind 1: PmmmDCT, M = the reduced length, K' = K(k) / (2M); kind 2: 3m with
+// zero translational parts, M = N / 2, K' = K(k) / (2M), k < M.  Layout:
+// cos, sin of 2 pi j / M (j < M, interleaved), cos, sin of pi k / 2M
+// (k < M, interleaved), then K'_k (k < M): 5 M values.
+std::vector<double> radix_table(int kind, int M, double L, double ds);
+
 // bf16 bits of x, round to nearest even (finite x).
 unsigned short bf16_bits(float x);
 

@@ -1,0 +1,392 @@
+// Radix FFT / DCT axis pass on gfx950 (LDS-resident, two lines per complex
+// transform).
+//
+// The same per-axis operator as the matrix passes (operators.hpp), applied
+// with fast transforms instead of a dense product:
+//
+//  - P1 (solver_p1.h:174-202): out = IFFT_N( K'(k) FFT_N(in) ), K' = K / N
+//    with k folded as k < (N+1)/2 ? k : N-k.  K' is real and even, so two
+//    real lines a, b go through one complex FFT of z = a + i b and come back
+//    as Re / Im of the inverse.
+//  - PmmmDCT (solver_Pmmm.h:194-215): out = REDFT01( K' REDFT10(in) ),
+//    K' = K / (2M), computed with M-point complex FFTs (Makhoul): the line x
+//    is reordered to v_j = x_{2j}, v_{M-1-j} = x_{2j+1} (j < M/2), V =
+//    FFT_M(v) gives REDFT10 as Y_k = 2 Re(c_k V_k), c_k = exp(-i pi k / 2M),
+//    and REDFT01 of Y' = K' Y is the reordered real part of the inverse FFT
+//    of W_k = conj(c_k) (Y'_k - i Y'_{M-k}).  With P = 2 c_k V_k =
+//    Y_k - i Y_{M-k} one step gives W_k = conj(c_k) (K'_k Re P + i K'_{M-k} Im P)
+//    from the pair (V_k, V_{M-k}); for two packed lines V_a, V_b are split
+//    from Z = FFT(v_a + i v_b) by Hermitian symmetry.
+//  - 3m with zero translational parts (solver_3m.h:281-324): the even
+//    sub-lattice e of a mirror-symmetric line (f(N-1-j) = f(j), N = 2M) is
+//    exactly the Makhoul order of the first half of f (v_j = f_{2j} = e_j,
+//    v_{M-1-j} = f_{2j+1} = f_{N-2-2j} = e_{M-1-j}), so the 3m operator is
+//    the PmmmDCT one without the reordering, with the same K'_k, k < M (the
+//    Nyquist mode of such a line vanishes).
+//
+// Each workgroup owns TL lines of one chain.  The tile is loaded coalesced
+// (with the optional pre multiply by E) straight into the transform layout:
+// packed line c holds line 2c in its real plane and line 2c+1 in its
+// imaginary plane, each point at its transform position (the Makhoul order
+// for PmmmDCT).  Every stage then works in place on that one LDS buffer:
+//
+//   forward  A: job (c, t), t < R2: DFT_R1 over the points t + R2 m, times
+//               w^(t k1), back to positions t + R2 k1;
+//   forward  B: job (c, s), s < R1: DFT_R2 over the positions s R2 + t; the
+//               output Z_(s + R1 k2) stays at position s R2 + k2;
+//   spectral:   the pairs (k, M-k) in that order;
+//   inverse  A: job (c, s): DFT_R2 over the positions s R2 + k2, times
+//               w^(j2 s), back to positions s R2 + j2;
+//   inverse  B: job (c, j2), j2 < R2: DFT_R1 over the positions s R2 + j2;
+//               output j1 goes to position j1 R2 + j2 = j, the natural order.
+//
+// Position p sits at p + p / R2 in its plane, so the R1 or R2 lanes of one
+// job group and the lanes of neighbouring jobs fall on distinct banks in
+// every stage.  The store reads the tile back from the transform layout
+// (with the optional post multiply).  Tiles are disjoint, so a pass may run
+// in place (out == in).
+#pragma once
+
+#include "axis_pass.hpp"
+
+namespace crys {
+
+enum RadixKind : int { kRadixP1 = 0, kRadixDct = 1, kRadixM3 = 2 };
+
+// Per-axis table (element type T) of a transform of length M:
+//   [0, 2M)   cos(2 pi j / M), sin(2 pi j / M) for j < M (interleaved)
+//   [2M, 4M)  cos(pi k / 2M), sin(pi k / 2M) for k < M (DCT kinds; P1: unused)
+//   [4M, 5M)  K'_k for k < M
+template <int M>
+struct RadixShape {
+  static constexpr int R1 = M <= 16 ? 4 : (M <= 64 ? 8 : 16);  // points per forward-A job
+  static constexpr int R2 = M / R1;                             // points per forward-B job
+  // lines per workgroup: 1024 points per tile (small tiles, so that a 64^3
+  // pass has 512 workgroups); 4096 at M = 128, whose x / y tiles then load
+  // rows of 32 contiguous values
+  static constexpr int TL = M == 128 ? 32 : 1024 / M;
+  static constexpr int NC = TL / 2;                             // packed complex lines
+  static constexpr int NT = NC * (R1 > R2 ? R1 : R2);           // threads
+  static constexpr int PP = M + R1 + 1;                         // plane pitch (odd)
+  static constexpr int TAB = 5 * M;
+  static_assert(M >= 8 && (M & (M - 1)) == 0 && M <= 128, "power-of-two length 8..128");
+  static_assert(R1 <= 16 && R2 <= 16 && R2 >= 2, "register radices");
+  static __device__ __forceinline__ int phys(int p) { return p + p / R2; }
+};
+
+template <typename T>
+struct RadixParams {
+  PassChain<T> c[2];
+  const T* tab;
+  int X, Y, Z;
+};
+
+// cos(2 pi p / 16) and sin(2 pi p / 16)
+__host__ __device__ constexpr double cos16(int p) {
+  constexpr double a = 0.92387953251128675613, b = 0.70710678118654752440,
+                   c = 0.38268343236508977173;
+  p &= 15;
+  return p == 0 ? 1.0 : p == 1 ? a : p == 2 ? b : p == 3 ? c : p == 4 ? 0.0 : p == 5 ? -c
+       : p == 6 ? -b : p == 7 ? -a : p == 8 ? -1.0 : p == 9 ? -a : p == 10 ? -b
+       : p == 11 ? -c : p == 12 ? 0.0 : p == 13 ? c : p == 14 ? b : a;
+}
+__host__ __device__ constexpr double sin16(int p) { return cos16(p - 4); }
+
+__host__ __device__ constexpr int bitrev(int i, int R) {
+  int r = 0;
+  for (int b = 1; b < R; b <<= 1) {
+    r = (r << 1) | (i & 1);
+    i >>= 1;
+  }
+  return r;
+}
+
+// In-register DFT of R <= 16 points: X_k = sum_m x_m exp(SIGN 2 pi i m k / R)
+// (radix-2, decimation in time; every twiddle is a compile-time constant).
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dft_reg(T (&re)[R], T (&im)[R]) {
+  T xr[R], xi[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    xr[bitrev(i, R)] = re[i];
+    xi[bitrev(i, R)] = im[i];
+  }
+#pragma unroll
+  for (int len = 2; len <= R; len <<= 1) {
+#pragma unroll
+    for (int s0 = 0; s0 < R; s0 += len) {
+#pragma unroll
+      for (int j = 0; j < len / 2; ++j) {
+        const int p = j * (16 / len);  // twiddle angle in units of 2 pi / 16
+        const int a = s0 + j, b = a + len / 2;
+        T br, bi;
+        if (p == 0) {
+          br = xr[b];
+          bi = xi[b];
+        } else if (p == 4) {  // exp(SIGN i pi / 2) = SIGN i
+          br = SIGN > 0 ? -xi[b] : xi[b];
+          bi = SIGN > 0 ? xr[b] : -xr[b];
+        } else {
+          const T wc = (T)cos16(p), ws = (T)(SIGN * sin16(p));
+          br = xr[b] * wc - xi[b] * ws;
+          bi = xr[b] * ws + xi[b] * wc;
+        }
+        xr[b] = xr[a] - br;
+        xi[b] = xi[a] - bi;
+        xr[a] = xr[a] + br;
+        xi[a] = xi[a] + bi;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    re[i] = xr[i];
+    im[i] = xi[i];
+  }
+}
+
+// One in-place stage over every packed line: job (c, g), g < G, loads the R
+// points at positions BASE_MUL g + STEP r (r < R) of both planes, applies
+// DFT_R with sign SIGN, multiplies output r by w_M^(SIGN g r) when `twiddle`
+// is set, and stores output r back to position BASE_MUL g + STEP r.
+template <typename T, int M, int R, int G, int SIGN, int STEP, int BASE_MUL>
+__device__ __forceinline__ void radix_stage(T* lds, const T* tw, bool twiddle) {
+  using S = RadixShape<M>;
+  for (int job = threadIdx.x; job < S::NC * G; job += S::NT) {
+    const int c = job / G, g = job - c * G;
+    T* const pr = lds + c * S::PP;
+    T* const pi = lds + (S::NC + c) * S::PP;
+    T vr[R], vi[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = S::phys(BASE_MUL * g + STEP * r);
+      vr[r] = pr[q];
+      vi[r] = pi[q];
+    }
+    dft_reg<T, R, SIGN>(vr, vi);
+    if (twiddle) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const int e = g * r;  // < M
+        const T wc = tw[2 * e], ws = SIGN > 0 ? tw[2 * e + 1] : -tw[2 * e + 1];
+        const T x = vr[r] * wc - vi[r] * ws;
+        vi[r] = vr[r] * ws + vi[r] * wc;
+        vr[r] = x;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = S::phys(BASE_MUL * g + STEP * r);
+      pr[q] = vr[r];
+      pi[q] = vi[r];
+    }
+  }
+}
+
+// Transform position of reduced index i (PmmmDCT: the Makhoul order
+// v_p = x_{2p}, v_{M-1-p} = x_{2p+1}); identity for P1 and 3m.
+template <int KIND, int M>
+__device__ __forceinline__ int radix_pos(int i) {
+  if constexpr (KIND == kRadixDct) return (i & 1) ? M - 1 - (i >> 1) : (i >> 1);
+  else return i;
+}
+
+template <typename T, int AXIS, int M, int KIND>
+__global__ __launch_bounds__(RadixShape<M>::NT) void radix_pass_kernel(RadixParams<T> p) {
+  using S = RadixShape<M>;
+  constexpr int NT = S::NT, TL = S::TL, NC = S::NC, PP = S::PP, R1 = S::R1, R2 = S::R2;
+  constexpr int KT = (M * TL + NT - 1) / NT;  // tile elements per thread
+  __shared__ T buf[2 * NC * PP];              // planes re (c < NC) | im
+  __shared__ T tab[S::TAB];
+
+  const PassChain<T> ch = p.c[blockIdx.y];
+  const int X = p.X, Y = p.Y, Z = p.Z;
+  const int YZ = Y * Z;
+  const int nlines = X * YZ / M;
+  const int l0 = blockIdx.x * TL;
+  const int nl = nlines - l0 < TL ? nlines - l0 : TL;
+  const int tid = threadIdx.x;
+  const bool yplane = AXIS == 1 && Z % TL == 0;
+  const int x0 = yplane ? l0 / Z : 0;
+  const int z0 = yplane ? l0 - x0 * Z : 0;
+
+  // ---- element (row i, line ll) of slot k: global index, LDS index, and
+  // whether it exists; recomputed for the store instead of held in registers
+  auto slot = [&](int k, int& g, int& l) -> bool {
+    const int f = tid + NT * k;
+    int i, ll;
+    if constexpr (AXIS == 2) {
+      ll = f / M;
+      i = f - ll * M;
+    } else {
+      i = f / TL;
+      ll = f - i * TL;
+    }
+    l = ((ll & 1) * NC + (ll >> 1)) * PP + S::phys(radix_pos<KIND, M>(i));
+    if constexpr (AXIS == 2) {
+      g = (l0 + ll) * Z + i;
+    } else if constexpr (AXIS == 1) {
+      if (yplane) {
+        g = x0 * YZ + i * Z + z0 + ll;
+      } else {
+        const int l1 = l0 + ll, x = l1 / Z;
+        g = x * YZ + i * Z + (l1 - x * Z);
+      }
+    } else {
+      g = i * YZ + l0 + ll;
+    }
+    return f < M * TL && ll < nl;
+  };
+
+  // ---- global loads first (tile, E, table), then LDS ----
+  {
+    int gi[KT], lo[KT];
+    bool ok[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      ok[k] = slot(k, gi[k], lo[k]);
+      if (!ok[k]) gi[k] = 0;
+    }
+    T v[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] = ok[k] ? ch.in[gi[k]] : T(0);
+    if (ch.pre) {
+      T e[KT];
+#pragma unroll
+      for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.pre[gi[k]] : T(0);
+#pragma unroll
+      for (int k = 0; k < KT; ++k) v[k] *= e[k];
+    }
+    for (int f = tid; f < S::TAB; f += NT) tab[f] = p.tab[f];
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (tid + NT * k < M * TL) buf[lo[k]] = v[k];
+  }
+  __syncthreads();
+
+  const T* const tw = tab;
+  const T* const cq = tab + 2 * M;
+  const T* const kp = tab + 4 * M;
+
+  // ---- forward FFT of z = line 2c + i line 2c+1 ----
+  radix_stage<T, M, R1, R2, -1, R2, 1>(buf, tw, true);   // A: points t + R2 m
+  __syncthreads();
+  radix_stage<T, M, R2, R1, -1, 1, R2>(buf, tw, false);  // B: points s R2 + t
+  __syncthreads();
+
+  // ---- spectral step; Z_k sits at position (k % R1) R2 + k / R1 ----
+  auto zpos = [](int k) { return S::phys((k % R1) * R2 + k / R1); };
+  if constexpr (KIND == kRadixP1) {
+    for (int f = tid; f < NC * M; f += NT) {
+      const int c = f / M, k = f - c * M;
+      const int q = zpos(k);
+      buf[c * PP + q] *= kp[k];
+      buf[(NC + c) * PP + q] *= kp[k];
+    }
+  } else {
+    constexpr int HP = M / 2 + 1;  // pairs (k, M-k), k = 0 .. M/2
+    for (int f = tid; f < NC * HP; f += NT) {
+      const int c = f / HP, k = f - c * HP;
+      const int km = (M - k) & (M - 1);
+      T* const pr = buf + c * PP;
+      T* const pi = buf + (NC + c) * PP;
+      const int q0 = zpos(k), q1 = zpos(km);
+      const T zr = pr[q0], zi = pi[q0];
+      const T yr = pr[q1], yi = -pi[q1];           // conj(Z_{M-k})
+      const T cr = cq[2 * k], ci = -cq[2 * k + 1];  // c_k = exp(-i pi k / 2M)
+      const T K0 = kp[k], K1 = kp[km];
+      // line a: P = c_k (Z_k + conj Z_{M-k}); line b: P = -i c_k (Z_k - conj Z_{M-k})
+      const T sr = zr + yr, si = zi + yi, dr = zr - yr, di = zi - yi;
+      const T par = cr * sr - ci * si, pai = cr * si + ci * sr;
+      const T qr = cr * dr - ci * di, qi = cr * di + ci * dr;
+      const T pbr = qi, pbi = -qr;
+      // W = conj(c_k) (K'_k Re P + i K'_{M-k} Im P)
+      const T ar = K0 * par, ai = K1 * pai, br = K0 * pbr, bi = K1 * pbi;
+      const T war = cr * ar + ci * ai, wai = cr * ai - ci * ar;
+      const T wbr = cr * br + ci * bi, wbi = cr * bi - ci * br;
+      // packed: W_k = Wa + i Wb, W_{M-k} = conj(Wa) + i conj(Wb)
+      pr[q0] = war - wbi;
+      pi[q0] = wai + wbr;
+      if (km != k) {
+        pr[q1] = war + wbi;
+        pi[q1] = wbr - wai;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- inverse FFT back to the natural order; Re / Im are the two lines ----
+  radix_stage<T, M, R2, R1, 1, 1, R2>(buf, tw, true);    // A': positions s R2 + k2
+  __syncthreads();
+  radix_stage<T, M, R1, R2, 1, R2, 1>(buf, tw, false);   // B': positions s R2 + j2
+  __syncthreads();
+
+  // ---- coalesced store, with the optional post multiply ----
+  int gi[KT], lo[KT];
+  bool ok[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    ok[k] = slot(k, gi[k], lo[k]);
+    if (!ok[k]) gi[k] = 0;
+  }
+  T v[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) v[k] = (tid + NT * k < M * TL) ? buf[lo[k]] : T(0);
+  if (ch.post) {
+    T e[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) e[k] = ok[k] ? ch.post[gi[k]] : T(0);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] *= e[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KT; ++k)
+    if (ok[k]) ch.out[gi[k]] = v[k];
+}
+
+// Whether an axis of transform length M can take the radix pass.
+inline bool radix_length_ok(int M) { return M >= 8 && M <= 128 && (M & (M - 1)) == 0; }
+
+template <typename T, int AXIS, int KIND>
+hipError_t launch_radix_pass_k(const RadixParams<T>& p, int M, int nchains, hipStream_t s) {
+  const long n = (long)p.X * p.Y * p.Z;
+  auto go = [&](auto mtag) -> hipError_t {
+    constexpr int MM = decltype(mtag)::value;
+    using S = RadixShape<MM>;
+    const long nlines = n / MM;
+    dim3 grid((unsigned)((nlines + S::TL - 1) / S::TL), (unsigned)nchains, 1);
+    hipLaunchKernelGGL((radix_pass_kernel<T, AXIS, MM, KIND>), grid, dim3(S::NT), 0, s, p);
+    return hipGetLastError();
+  };
+  switch (M) {
+    case 8: return go(std::integral_constant<int, 8>{});
+    case 16: return go(std::integral_constant<int, 16>{});
+    case 32: return go(std::integral_constant<int, 32>{});
+    case 64: return go(std::integral_constant<int, 64>{});
+    case 128: return go(std::integral_constant<int, 128>{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// One radix pass along `axis` (transform length = the axis length) for 1 or
+// 2 chains.
+template <typename T>
+hipError_t launch_radix_pass(int axis, int kind, const RadixParams<T>& p, int nchains,
+                             hipStream_t s) {
+  if (nchains < 1 || nchains > 2) return hipErrorInvalidValue;
+  const int M = axis == 0 ? p.X : (axis == 1 ? p.Y : p.Z);
+  if (!radix_length_ok(M)) return hipErrorInvalidValue;
+#define CRYS_RADIX_AX(A)                                                              \
+  if (axis == A) {                                                                    \
+    if (kind == kRadixP1) return launch_radix_pass_k<T, A, kRadixP1>(p, M, nchains, s); \
+    if (kind == kRadixDct) return launch_radix_pass_k<T, A, kRadixDct>(p, M, nchains, s); \
+    return launch_radix_pass_k<T, A, kRadixM3>(p, M, nchains, s);                      \
+  }
+  CRYS_RADIX_AX(0)
+  CRYS_RADIX_AX(1)
+  CRYS_RADIX_AX(2)
+#undef CRYS_RADIX_AX
+  return hipErrorInvalidValue;
+}
+
+}  // namespace crys
