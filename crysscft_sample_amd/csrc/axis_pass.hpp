@@ -562,11 +562,13 @@ struct SlabOperands {
   int brow[KS];  // LDS row of the B operand of this lane for each step
 };
 
+// `pitch` (> 0) overrides the row length of the stored operator, for a copy
+// held in LDS at a padded pitch.
 template <typename T, int MP, bool CEN>
 __device__ __forceinline__ void slab_load_operands(const T* __restrict__ mat,
                                                    const unsigned short* __restrict__ lo, int M,
                                                    int jt, int lane,
-                                                   SlabOperands<T, MP, CEN>& op) {
+                                                   SlabOperands<T, MP, CEN>& op, int pitch = 0) {
   using S = SlabShape<MP, CEN>;
   constexpr int KS = S::KS, JT = S::JT;
   constexpr bool SPLIT = std::is_same<T, float>::value;
@@ -577,7 +579,7 @@ __device__ __forceinline__ void slab_load_operands(const T* __restrict__ mat,
   int ord[KS];
   far_first_order<KS>(4.0f * jb + 1.5f, ord);
   const int jrow = jt * 16 + (lane & 15);
-  const int ldm = CEN ? MP : M;  // row length of the stored operator
+  const int ldm = pitch > 0 ? pitch : (CEN ? MP : M);  // row length of the stored operator
 #pragma unroll
   for (int n = 0; n < KS; ++n) {
     const int i = 4 * ord[n] + (lane >> 4);  // k index inside the k space

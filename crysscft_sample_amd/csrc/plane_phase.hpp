@@ -483,7 +483,8 @@ struct KHalfOperands {
 template <typename T, int MP>
 __device__ __forceinline__ void load_operands_khalf(const T* __restrict__ mat,
                                                     const unsigned short* __restrict__ lo, int jt,
-                                                    int lane, int kh, KHalfOperands<T, MP>& op) {
+                                                    int lane, int kh, KHalfOperands<T, MP>& op,
+                                                    int pitch = MP) {
   constexpr int KS = MP / 8, JT = MP / 16;
   static_assert(KS == 8, "k halves of 4 steps");
   constexpr bool SPLIT = std::is_same<T, float>::value;
@@ -498,8 +499,8 @@ __device__ __forceinline__ void load_operands_khalf(const T* __restrict__ mat,
     const int o = kh ? ord[4 + i] : ord[i];
     const int k = 4 * o + (lane >> 4);
     op.brow[i] = kbase + k;
-    op.a[i] = mat[k * MP + jrow];
-    if constexpr (SPLIT) op.alo[i] = lo ? (short)lo[k * MP + jrow] : (short)0;
+    op.a[i] = mat[k * pitch + jrow];
+    if constexpr (SPLIT) op.alo[i] = lo ? (short)lo[k * pitch + jrow] : (short)0;
   }
 }
 
@@ -582,6 +583,16 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   __shared__ T lds2[2 * MP * PB];
   T* const buf0 = lds2;
   T* const buf1 = lds2 + MP * PB;
+  // FP64: the row and z operators (half-size, H x MP) are loaded once per
+  // workgroup into LDS (pitch OP: the four k rows of one read fall on
+  // alternate bank halves), and each product reads its operator registers
+  // from there: the waves' registers repeat each operator tile 2 to 4 times,
+  // 128 KB per workgroup from global memory otherwise, which delays the
+  // plane.  FP32 operators are half the bytes and load straight into
+  // registers (measured faster for FP32, slower for FP64).
+  constexpr bool OPS_LDS = !SPLIT;
+  constexpr int OP = MP + 16;
+  __shared__ T opl[OPS_LDS ? 2 * H * OP : 1];
 
   const int id = blockIdx.x;
   const int h = (id >> 3) & 1;
@@ -602,11 +613,22 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   const int rt3 = (item & 1) ? 3 - h : h;
   SlabOperands<T, MP, true> o1;
   KHalfOperands<T, MP> o2, o3;
-  slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, w % JT, lane, o1);
-  load_operands_khalf<T, MP>(p.mat_r, p.lo_r, r2t, lane, kh, o2);
-  load_operands_khalf<T, MP>(p.mat_c, p.lo_c, ct2, lane, kh, o3);
 
-  // ---- global loads up front: the plane and its E values ----
+  // ---- global loads up front: the operators, the plane and its E values ----
+  constexpr int KO = H * MP / NT;
+  static_assert(KO * NT == H * MP, "operator copy splits evenly");
+  T om[2][OPS_LDS ? KO : 1];
+  if constexpr (OPS_LDS) {
+#pragma unroll
+    for (int k = 0; k < KO; ++k) {
+      om[0][k] = p.mat_r[tid + NT * k];
+      om[1][k] = p.mat_c[tid + NT * k];
+    }
+  } else {
+    slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, w % JT, lane, o1);
+    load_operands_khalf<T, MP>(p.mat_r, p.lo_r, r2t, lane, kh, o2);
+    load_operands_khalf<T, MP>(p.mat_c, p.lo_c, ct2, lane, kh, o3);
+  }
   constexpr int KE = H * MP / NT;
   long g1[KE], g2[KE];
   T x1[KE], x2[KE], ef1[KE], ef2[KE];
@@ -632,7 +654,15 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     ef2[k] = e_a[g2[k]];
   }
 
-  // ---- stage u/v rows (start: times E of the next step) ----
+  // ---- operators to LDS; stage u/v rows (start: times E of the next step) ----
+  if constexpr (OPS_LDS) {
+#pragma unroll
+    for (int k = 0; k < KO; ++k) {
+      const int f = tid + NT * k, r = f / MP, c = f - r * MP;
+      opl[r * OP + c] = om[0][k];
+      opl[H * OP + r * OP + c] = om[1][k];
+    }
+  }
   if (fin) {
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
@@ -654,6 +684,7 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   __syncthreads();
 
   if (fin) {
+    if constexpr (OPS_LDS) slab_load_operands<T, MP, true>(opl, nullptr, MP, w % JT, lane, o1, OP);
     // ---- finishing row product, whole plane, one tile per wave ----
     V4 acc[TPW];
     phase_row_product<T, MP, true, NT>(o1, lo_r, buf0, w, lane, acc);
@@ -691,6 +722,7 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   }
 
   // ---- next row product: U/V row tiles h and 2 + h, k halves ----
+  if constexpr (OPS_LDS) load_operands_khalf<T, MP>(opl, nullptr, r2t, lane, kh, o2, OP);
   {
     const V4 acc = mfma_tile_b_khalf<T, MP>(o2, lo_r, buf0, PB, ct2 * 16 + (lane & 15));
     const int rt = kh ? (r2t ^ 1) : r2t;
@@ -721,6 +753,7 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   }
   __syncthreads();
   // ---- z product: output row tiles h and 3 - h, k halves ----
+  if constexpr (OPS_LDS) load_operands_khalf<T, MP>(opl + H * OP, nullptr, ct2, lane, kh, o3, OP);
   {
     const V4 acc = mfma_tile_a_khalf<T, MP>(o3, lo_c, buf0, PA, rt3 * 16 + (lane & 15));
     const int rt = kh ? (rt3 ^ 1) : rt3;
