@@ -287,7 +287,7 @@ PASS_PATHS = {"phase": {}, "slab+x": {"CRYS_PHASE": "0"},
               "radix_passes": {"CRYS_PHASE": "0", "CRYS_AXIS_PASS": "radix"}}
 
 
-def passes_timing(iters):
+def passes_timing(iters, only=None):
     """One JSON line: ms per contour step (both chains) of each kernel path
     of the SCFT iteration, on the headline grid and on two grids of axis
     length 128: the one-launch plane phase, the fused z+y slab plus the x
@@ -299,6 +299,8 @@ def passes_timing(iters):
     out = {"metric": "ms per contour step (both chains) by kernel path, SCFT iteration, Ns=200",
            "unit": "ms", "higher_is_better": False, "grids": {}}
     for name, N, L in PASS_GRIDS:
+        if only and f"{name}:{N}" not in only:
+            continue
         n = (N // 2) ** 3 if name != "P1" else N ** 3
         g = {}
         for prec in ("fp64", "fp32"):
@@ -341,6 +343,8 @@ def main():
     ap.add_argument("--passes", action="store_true",
                     help="time the contour step of every kernel path (phase, slab, matrix-core "
                          "passes, radix passes) and exit")
+    ap.add_argument("--pass-grid", action="append", metavar="SOLVER:N",
+                    help="with --passes: time only these grids (e.g. PmmmDCT:256)")
     ap.add_argument("--batch", type=int, default=0, metavar="COUNT",
                     help="time COUNT batched instances (the batching workload) and exit")
     args = ap.parse_args()
@@ -357,7 +361,7 @@ def main():
     if args.triplet:
         return triplet_timing(200)
     if args.passes:
-        return passes_timing(5)
+        return passes_timing(5, args.pass_grid)
 
     from crysscft_sample_amd import fields as F
 
