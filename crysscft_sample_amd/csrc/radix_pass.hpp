@@ -69,6 +69,7 @@ struct RadixShape {
   static constexpr int NT = NC * (R1 > R2 ? R1 : R2);           // threads
   static constexpr int PP = M + R1 + 1;                         // plane pitch (odd)
   static constexpr int TAB = 5 * M;
+  static constexpr int TAB_LDS = 3 * M;  // FFT twiddles and K' (the c_k stay in global memory)
   static_assert(M >= 8 && (M & (M - 1)) == 0 && M <= 128, "power-of-two length 8..128");
   static_assert(R1 <= 16 && R2 <= 16 && R2 >= 2, "register radices");
   static __device__ __forceinline__ int phys(int p) { return p + p / R2; }
@@ -191,13 +192,23 @@ __device__ __forceinline__ int radix_pos(int i) {
   else return i;
 }
 
+// Waves per SIMD the compiler must allow: 4 (four 256-thread workgroups per
+// CU) where that fits in 128 VGPRs without spilling; the FP64 length-128
+// kernel needs about 150 and measured slower when held to 128 (spills).
+template <typename T, int M>
+constexpr int radix_waves_per_eu() {
+  return (sizeof(T) == 8 && M == 128) ? 1 : 4;
+}
+
 template <typename T, int AXIS, int M, int KIND>
-__global__ __launch_bounds__(RadixShape<M>::NT) void radix_pass_kernel(RadixParams<T> p) {
+__global__ __launch_bounds__(RadixShape<M>::NT)
+__attribute__((amdgpu_waves_per_eu(radix_waves_per_eu<T, M>()))) void radix_pass_kernel(
+    RadixParams<T> p) {
   using S = RadixShape<M>;
   constexpr int NT = S::NT, TL = S::TL, NC = S::NC, PP = S::PP, R1 = S::R1, R2 = S::R2;
   constexpr int KT = (M * TL + NT - 1) / NT;  // tile elements per thread
   __shared__ T buf[2 * NC * PP];              // planes re (c < NC) | im
-  __shared__ T tab[S::TAB];
+  __shared__ T tab[S::TAB_LDS];  // FFT twiddles (2M), then K' (M)
 
   const PassChain<T> ch = p.c[blockIdx.y];
   const int X = p.X, Y = p.Y, Z = p.Z;
@@ -257,7 +268,7 @@ __global__ __launch_bounds__(RadixShape<M>::NT) void radix_pass_kernel(RadixPara
 #pragma unroll
       for (int k = 0; k < KT; ++k) v[k] *= e[k];
     }
-    for (int f = tid; f < S::TAB; f += NT) tab[f] = p.tab[f];
+    for (int f = tid; f < S::TAB_LDS; f += NT) tab[f] = p.tab[f < 2 * M ? f : f + 2 * M];
 #pragma unroll
     for (int k = 0; k < KT; ++k)
       if (tid + NT * k < M * TL) buf[lo[k]] = v[k];
@@ -265,8 +276,8 @@ __global__ __launch_bounds__(RadixShape<M>::NT) void radix_pass_kernel(RadixPara
   __syncthreads();
 
   const T* const tw = tab;
-  const T* const cq = tab + 2 * M;
-  const T* const kp = tab + 4 * M;
+  const T* const cq = p.tab + 2 * M;
+  const T* const kp = tab + 2 * M;
 
   // ---- forward FFT of z = line 2c + i line 2c+1 ----
   radix_stage<T, M, R1, R2, -1, R2, 1>(buf, tw, true);   // A: points t + R2 m
