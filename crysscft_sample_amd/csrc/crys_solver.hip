@@ -307,7 +307,18 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
   };
   crys::PassChain<T> cc[2];
   int k = 0;
-  if (crys::slab_applies(Y, Z) && !op.drad[1] && !op.drad[2]) {
+  if (op.drad[1] && op.drad[2] && crys::radix_slab_applies(Y, Z) && crys::slab_enabled()) {
+    // z and y radix transforms fused on x-slabs (both axes of length 128)
+    crys::RadixSlabParams<T> rs{};
+    rs.X = X;
+    rs.Y = Y;
+    rs.Z = Z;
+    rs.tab_z = (const T*)op.drad[2];
+    rs.tab_y = (const T*)op.drad[1];
+    for (int c = 0; c < nch; ++c) rs.c[c] = {in[c], scr[c], E[c], nullptr};
+    if (only < 0 || only == k) HIPCHK(crys::launch_radix_slab<T>(radix_kind(h), rs, nch, s));
+    ++k;
+  } else if (crys::slab_applies(Y, Z) && !op.drad[1] && !op.drad[2]) {
     // z and y fused on x-slabs when the two axes share a length class; the
     // half-size centrosymmetric operators when both axes have them
     crys::SlabParams<T> sp{};

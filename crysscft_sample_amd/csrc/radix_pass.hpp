@@ -355,6 +355,209 @@ __attribute__((amdgpu_waves_per_eu(radix_waves_per_eu<T, M>()))) void radix_pass
     if (ok[k]) ch.out[gi[k]] = v[k];
 }
 
+// ---------------------------------------------------------------------------
+// Fused z + y radix slab at length 128: one workgroup holds the whole x-plane
+// (Y = Z = 128) in LDS as a real array pl[row][col], row = transform position
+// of y, column = padded transform position of z.  The z transforms pack two
+// rows as the real and imaginary parts of one complex line, the y transforms
+// two columns; every stage works in place on that one array, with the stage
+// order and position bookkeeping of the pass kernel above.  A contour step is
+// then this kernel (in -> scr, pre E) and one radix x pass (scr -> out, post
+// E): 6 n b per chain instead of the 8 n b of three passes.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RadixSlabParams {
+  PassChain<T> c[2];
+  const T* tab_z;  // radix_table of the z axis
+  const T* tab_y;  // radix_table of the y axis
+  int X, Y, Z;
+};
+
+template <typename T>
+struct RadixSlabShape {
+  static constexpr int M = 128, R1 = 16, R2 = 8;
+  // one workgroup per CU (the plane fills the LDS): 1024 threads for FP32,
+  // 512 for FP64, whose 16-point stages need more than 128 VGPRs
+  static constexpr int NT = sizeof(T) == 8 ? 512 : 1024;
+  static constexpr int NCL = M / 2;           // packed lines per direction
+  static constexpr int RP = 148;              // row pitch: >= M + M / R2, = 4 mod 16
+  static_assert(RP >= M + M / R2 && RP % 16 == 4, "row pitch");
+  static __device__ __forceinline__ int col(int q) { return q + q / R2; }
+};
+
+// In-place stage over the packed lines of one direction (see radix_stage):
+// job (c, g), g < G; DFT_R over the positions BASE_MUL g + STEP r; twiddle
+// w_M^(SIGN g r) when `twiddle`.  Z_DIR: line c = rows 2c, 2c+1 (positions
+// along the row, jobs of one line on neighbouring lanes); else line c =
+// columns 2c, 2c+1 (positions down the column, lines on neighbouring lanes).
+template <typename T, bool Z_DIR, int R, int G, int SIGN, int STEP, int BASE_MUL>
+__device__ __forceinline__ void slab_stage(T* pl, const T* tw, bool twiddle) {
+  using S = RadixSlabShape<T>;
+  constexpr int M = S::M, NCL = S::NCL, RP = S::RP;
+  for (int job = threadIdx.x; job < NCL * G; job += S::NT) {
+    const int c = Z_DIR ? job / G : job % NCL;
+    const int g = Z_DIR ? job - c * G : job / NCL;
+    auto at = [&](int pos, int part) {
+      return Z_DIR ? (2 * c + part) * RP + S::col(pos) : pos * RP + S::col(2 * c + part);
+    };
+    T vr[R], vi[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = BASE_MUL * g + STEP * r;
+      vr[r] = pl[at(q, 0)];
+      vi[r] = pl[at(q, 1)];
+    }
+    dft_reg<T, R, SIGN>(vr, vi);
+    if (twiddle) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const int e = g * r;  // < M
+        const T wc = tw[2 * e], ws = SIGN > 0 ? tw[2 * e + 1] : -tw[2 * e + 1];
+        const T x = vr[r] * wc - vi[r] * ws;
+        vi[r] = vr[r] * ws + vi[r] * wc;
+        vr[r] = x;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = BASE_MUL * g + STEP * r;
+      pl[at(q, 0)] = vr[r];
+      pl[at(q, 1)] = vi[r];
+    }
+  }
+  (void)M;
+}
+
+// Forward FFT, spectral step (K' of this direction; cq the DCT twiddles in
+// global memory) and inverse FFT of every packed line of one direction.
+template <typename T, bool Z_DIR, int KIND>
+__device__ __forceinline__ void slab_transform(T* pl, const T* tw, const T* kp, const T* cq) {
+  using S = RadixSlabShape<T>;
+  constexpr int M = S::M, R1 = S::R1, R2 = S::R2, NCL = S::NCL, RP = S::RP, NT = S::NT;
+  slab_stage<T, Z_DIR, R1, R2, -1, R2, 1>(pl, tw, true);   // A: points t + R2 m
+  __syncthreads();
+  slab_stage<T, Z_DIR, R2, R1, -1, 1, R2>(pl, tw, false);  // B: points s R2 + t
+  __syncthreads();
+  // Z_k sits at position (k % R1) R2 + k / R1
+  auto at = [&](int c, int k, int part) {
+    const int pos = (k % R1) * R2 + k / R1;
+    return Z_DIR ? (2 * c + part) * RP + S::col(pos) : pos * RP + S::col(2 * c + part);
+  };
+  if constexpr (KIND == kRadixP1) {
+    for (int f = threadIdx.x; f < NCL * M; f += NT) {
+      const int c = Z_DIR ? f / M : f % NCL, k = Z_DIR ? f - c * M : f / NCL;
+      pl[at(c, k, 0)] *= kp[k];
+      pl[at(c, k, 1)] *= kp[k];
+    }
+  } else {
+    constexpr int HP = M / 2 + 1;
+    for (int f = threadIdx.x; f < NCL * HP; f += NT) {
+      const int c = Z_DIR ? f / HP : f % NCL, k = Z_DIR ? f - c * HP : f / NCL;
+      const int km = (M - k) & (M - 1);
+      const int a0 = at(c, k, 0), a1 = at(c, k, 1), b0 = at(c, km, 0), b1 = at(c, km, 1);
+      const T zr = pl[a0], zi = pl[a1];
+      const T yr = pl[b0], yi = -pl[b1];            // conj(Z_{M-k})
+      const T cr = cq[2 * k], ci = -cq[2 * k + 1];  // c_k = exp(-i pi k / 2M)
+      const T K0 = kp[k], K1 = kp[km];
+      const T sr = zr + yr, si = zi + yi, dr = zr - yr, di = zi - yi;
+      const T par = cr * sr - ci * si, pai = cr * si + ci * sr;
+      const T qr = cr * dr - ci * di, qi = cr * di + ci * dr;
+      const T pbr = qi, pbi = -qr;
+      const T ar = K0 * par, ai = K1 * pai, br = K0 * pbr, bi = K1 * pbi;
+      const T war = cr * ar + ci * ai, wai = cr * ai - ci * ar;
+      const T wbr = cr * br + ci * bi, wbi = cr * bi - ci * br;
+      pl[a0] = war - wbi;
+      pl[a1] = wai + wbr;
+      if (km != k) {
+        pl[b0] = war + wbi;
+        pl[b1] = wbr - wai;
+      }
+    }
+  }
+  __syncthreads();
+  slab_stage<T, Z_DIR, R2, R1, 1, 1, R2>(pl, tw, true);    // A': positions s R2 + k2
+  __syncthreads();
+  slab_stage<T, Z_DIR, R1, R2, 1, R2, 1>(pl, tw, false);   // B': positions s R2 + j2
+  __syncthreads();
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(RadixSlabShape<T>::NT) void radix_slab_kernel(RadixSlabParams<T> p) {
+  using S = RadixSlabShape<T>;
+  constexpr int M = S::M, NT = S::NT, RP = S::RP;
+  constexpr int KT = M * M / NT;  // plane elements per thread
+  constexpr int KB = 8;           // loaded in batches of 8 (registers)
+  constexpr int NBAT = KT / KB;
+  static_assert(NBAT * KB == KT, "batches");
+  __shared__ T pl[M * RP];
+  __shared__ T tab[4 * M];  // FFT twiddles (2M), K' of z (M), K' of y (M)
+
+  const PassChain<T> ch = p.c[blockIdx.y];
+  const long base = (long)blockIdx.x * M * M;
+  const int tid = threadIdx.x;
+  auto lds_at = [](int f) {
+    const int y = f / M, z = f - y * M;
+    return radix_pos<KIND, M>(y) * RP + S::col(radix_pos<KIND, M>(z));
+  };
+
+  for (int f = tid; f < 4 * M; f += NT)
+    tab[f] = f < 2 * M ? p.tab_z[f] : (f < 3 * M ? p.tab_z[f + 2 * M] : p.tab_y[f + M]);
+#pragma unroll
+  for (int b = 0; b < NBAT; ++b) {
+    T v[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) v[k] = ch.in[base + tid + NT * (b * KB + k)];
+    if (ch.pre) {
+      T e[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) e[k] = ch.pre[base + tid + NT * (b * KB + k)];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) v[k] *= e[k];
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k) pl[lds_at(tid + NT * (b * KB + k))] = v[k];
+  }
+  __syncthreads();
+
+  slab_transform<T, true, KIND>(pl, tab, tab + 2 * M, p.tab_z + 2 * M);   // z
+  slab_transform<T, false, KIND>(pl, tab, tab + 3 * M, p.tab_y + 2 * M);  // y
+
+#pragma unroll
+  for (int b = 0; b < NBAT; ++b) {
+    T v[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) v[k] = pl[lds_at(tid + NT * (b * KB + k))];
+    if (ch.post) {
+      T e[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) e[k] = ch.post[base + tid + NT * (b * KB + k)];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) v[k] *= e[k];
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k) ch.out[base + tid + NT * (b * KB + k)] = v[k];
+  }
+}
+
+// Whether the fused z + y radix slab applies: both axes of length 128.
+inline bool radix_slab_applies(int Y, int Z) { return Y == 128 && Z == 128; }
+
+// One z + y radix slab (in -> out per chain, optional pre / post E) for 1 or
+// 2 chains; tab_z and tab_y are the radix tables of the z and y axes.
+template <typename T>
+hipError_t launch_radix_slab(int kind, const RadixSlabParams<T>& p, int nchains, hipStream_t s) {
+  if (nchains < 1 || nchains > 2 || !radix_slab_applies(p.Y, p.Z)) return hipErrorInvalidValue;
+  dim3 grid((unsigned)p.X, (unsigned)nchains, 1);
+  const dim3 block(RadixSlabShape<T>::NT);
+  if (kind == kRadixP1)
+    hipLaunchKernelGGL((radix_slab_kernel<T, kRadixP1>), grid, block, 0, s, p);
+  else if (kind == kRadixDct)
+    hipLaunchKernelGGL((radix_slab_kernel<T, kRadixDct>), grid, block, 0, s, p);
+  else
+    hipLaunchKernelGGL((radix_slab_kernel<T, kRadixM3>), grid, block, 0, s, p);
+  return hipGetLastError();
+}
+
 // Whether an axis of transform length M can take the radix pass.
 inline bool radix_length_ok(int M) { return M >= 8 && M <= 128 && (M & (M - 1)) == 0; }
 
