@@ -353,9 +353,15 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
   return CRYS_OK;
 }
 
-// Length class of the one-launch phase path for this solver (0: not used).
+// Length class of the one-launch phase path for this solver (0: not used):
+// the matrix-core plane phases when all three axes share a length class up
+// to 64, the radix plane phase when all three axes have length 128 and take
+// the radix pass.
 int phase_mp(const crys_solver* h) {
   if (!crys::phase_enabled()) return 0;
+  if (h->nphy[0] == 128 && h->nphy[1] == 128 && h->nphy[2] == 128 && radix_axis(h, 0) &&
+      radix_axis(h, 1) && radix_axis(h, 2))
+    return 128;
   return crys::phase_class((int)h->nphy[0], (int)h->nphy[1], (int)h->nphy[2]);
 }
 
@@ -408,6 +414,13 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
     f.e_next = Ef(k);
     b.out = scr[1] + wr;
     b.e_next = Eb(k);
+  }
+  if (mp == 128) {
+    p.tab_r = (const T*)op.drad[ra];
+    p.tab_c = (const T*)op.drad[2];
+    p.rkind = radix_kind(h);
+    HIPCHK(crys::launch_radix_phase<T>(p, nplanes, nch, s));
+    return CRYS_OK;
   }
   HIPCHK(crys::launch_phase<T>(p, nplanes, nch, mp, cen, s));
   return CRYS_OK;

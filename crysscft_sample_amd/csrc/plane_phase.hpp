@@ -50,6 +50,9 @@ struct PhaseParams {
   PhaseChain<T> c[2];
   const T* mat_r;  // row-axis operator: dense At (M x M) or At_c (M/2 x M)
   const T* mat_c;  // z operator
+  const T* tab_r;  // radix tables of the row axis and z (length-128 radix phase), or null
+  const T* tab_c;
+  int rkind;       // RadixKind of the radix phase
   const unsigned short* lo_r;  // bf16 low parts (FP32), or null
   const unsigned short* lo_c;
   int MR, MC;           // row axis length, z length

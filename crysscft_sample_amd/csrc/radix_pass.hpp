@@ -48,6 +48,7 @@
 #pragma once
 
 #include "axis_pass.hpp"
+#include "plane_phase.hpp"
 
 namespace crys {
 
@@ -390,11 +391,11 @@ struct RadixSlabShape {
 // w_M^(SIGN g r) when `twiddle`.  Z_DIR: line c = rows 2c, 2c+1 (positions
 // along the row, jobs of one line on neighbouring lanes); else line c =
 // columns 2c, 2c+1 (positions down the column, lines on neighbouring lanes).
-template <typename T, bool Z_DIR, int R, int G, int SIGN, int STEP, int BASE_MUL>
+template <typename T, int NT, bool Z_DIR, int R, int G, int SIGN, int STEP, int BASE_MUL>
 __device__ __forceinline__ void slab_stage(T* pl, const T* tw, bool twiddle) {
   using S = RadixSlabShape<T>;
   constexpr int M = S::M, NCL = S::NCL, RP = S::RP;
-  for (int job = threadIdx.x; job < NCL * G; job += S::NT) {
+  for (int job = threadIdx.x; job < NCL * G; job += NT) {
     const int c = Z_DIR ? job / G : job % NCL;
     const int g = Z_DIR ? job - c * G : job / NCL;
     auto at = [&](int pos, int part) {
@@ -430,13 +431,16 @@ __device__ __forceinline__ void slab_stage(T* pl, const T* tw, bool twiddle) {
 
 // Forward FFT, spectral step (K' of this direction; cq the DCT twiddles in
 // global memory) and inverse FFT of every packed line of one direction.
-template <typename T, bool Z_DIR, int KIND>
-__device__ __forceinline__ void slab_transform(T* pl, const T* tw, const T* kp, const T* cq) {
+// NT: the threads of the calling workgroup; `before_last()` runs before the
+// last stage (a caller's loads that should be in flight during it).
+template <typename T, int NT, bool Z_DIR, int KIND, typename F>
+__device__ __forceinline__ void slab_transform(T* pl, const T* tw, const T* kp, const T* cq,
+                                               F before_last) {
   using S = RadixSlabShape<T>;
-  constexpr int M = S::M, R1 = S::R1, R2 = S::R2, NCL = S::NCL, RP = S::RP, NT = S::NT;
-  slab_stage<T, Z_DIR, R1, R2, -1, R2, 1>(pl, tw, true);   // A: points t + R2 m
+  constexpr int M = S::M, R1 = S::R1, R2 = S::R2, NCL = S::NCL, RP = S::RP;
+  slab_stage<T, NT, Z_DIR, R1, R2, -1, R2, 1>(pl, tw, true);   // A: points t + R2 m
   __syncthreads();
-  slab_stage<T, Z_DIR, R2, R1, -1, 1, R2>(pl, tw, false);  // B: points s R2 + t
+  slab_stage<T, NT, Z_DIR, R2, R1, -1, 1, R2>(pl, tw, false);  // B: points s R2 + t
   __syncthreads();
   // Z_k sits at position (k % R1) R2 + k / R1
   auto at = [&](int c, int k, int part) {
@@ -475,10 +479,16 @@ __device__ __forceinline__ void slab_transform(T* pl, const T* tw, const T* kp, 
     }
   }
   __syncthreads();
-  slab_stage<T, Z_DIR, R2, R1, 1, 1, R2>(pl, tw, true);    // A': positions s R2 + k2
+  slab_stage<T, NT, Z_DIR, R2, R1, 1, 1, R2>(pl, tw, true);    // A': positions s R2 + k2
   __syncthreads();
-  slab_stage<T, Z_DIR, R1, R2, 1, R2, 1>(pl, tw, false);   // B': positions s R2 + j2
+  before_last();
+  slab_stage<T, NT, Z_DIR, R1, R2, 1, R2, 1>(pl, tw, false);   // B': positions s R2 + j2
   __syncthreads();
+}
+
+template <typename T, int NT, bool Z_DIR, int KIND>
+__device__ __forceinline__ void slab_transform(T* pl, const T* tw, const T* kp, const T* cq) {
+  slab_transform<T, NT, Z_DIR, KIND>(pl, tw, kp, cq, [] {});
 }
 
 template <typename T, int KIND>
@@ -519,8 +529,8 @@ __global__ __launch_bounds__(RadixSlabShape<T>::NT) void radix_slab_kernel(Radix
   }
   __syncthreads();
 
-  slab_transform<T, true, KIND>(pl, tab, tab + 2 * M, p.tab_z + 2 * M);   // z
-  slab_transform<T, false, KIND>(pl, tab, tab + 3 * M, p.tab_y + 2 * M);  // y
+  slab_transform<T, NT, true, KIND>(pl, tab, tab + 2 * M, p.tab_z + 2 * M);   // z
+  slab_transform<T, NT, false, KIND>(pl, tab, tab + 3 * M, p.tab_y + 2 * M);  // y
 
 #pragma unroll
   for (int b = 0; b < NBAT; ++b) {
@@ -555,6 +565,114 @@ hipError_t launch_radix_slab(int kind, const RadixSlabParams<T>& p, int nchains,
     hipLaunchKernelGGL((radix_slab_kernel<T, kRadixDct>), grid, block, 0, s, p);
   else
     hipLaunchKernelGGL((radix_slab_kernel<T, kRadixM3>), grid, block, 0, s, p);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// One launch per contour step at length 128 (all three axes): the plane phase
+// of plane_phase.hpp with radix transforms.  A workgroup holds one plane of
+// one chain (rows = the row axis, y for x-slabs and x for y-slabs, columns =
+// z) in LDS as in the slab kernel above, finishes the previous step along the
+// row axis, multiplies by E, writes the history entry, multiplies by E of the
+// next step, applies the z and row-axis transforms of the next step and
+// writes the scratch plane for the next phase: 4 n b per chain and step.
+// 512 threads; each thread keeps the E values of its 32 plane elements in
+// registers from the start, so the E stage waits for no load.
+// ---------------------------------------------------------------------------
+template <typename T, int KIND>
+__global__ __launch_bounds__(512) void radix_phase_kernel(PhaseParams<T> p) {
+  using S = RadixSlabShape<T>;
+  constexpr int M = S::M, NT = 512, RP = S::RP;
+  constexpr int KT = M * M / NT;  // plane elements per thread
+  __shared__ T pl[M * RP];
+  __shared__ T tab[4 * M];  // FFT twiddles (2M), K' of z (M), K' of the row axis (M)
+
+  const PhaseChain<T> ch = p.c[blockIdx.y];
+  const int RS = (int)p.row_stride;  // < 2^31 / 128 for the grids this kernel runs
+  const long base = (long)blockIdx.x * p.plane_stride;
+  const int tid = threadIdx.x;
+  const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
+  // 32-bit offsets from per-plane base pointers (one register per address)
+  const T* const in_b = ch.in + base;
+  const T* const efin_b = fin ? ch.e_fin + base : nullptr;
+  const T* const enext_b = next ? ch.e_next + base : nullptr;
+  T* const hist_b = fin ? ch.hist + base : nullptr;
+  T* const out_b = next ? ch.out + base : nullptr;
+  auto gidx = [&](int f) {
+    const int r = f / M, z = f - r * M;
+    return r * RS + z;
+  };
+  auto lds_at = [](int f) {
+    const int r = f / M, z = f - r * M;
+    return radix_pos<KIND, M>(r) * RP + S::col(radix_pos<KIND, M>(z));
+  };
+
+  for (int f = tid; f < 4 * M; f += NT)
+    tab[f] = f < 2 * M ? p.tab_c[f] : (f < 3 * M ? p.tab_c[f + 2 * M] : p.tab_r[f + M]);
+  // the plane and E (the finished step's; the next step's in a start phase),
+  // both in the coalesced element order f = tid + NT k.  FP64 loads the
+  // finished step's E only before the last stage of the finishing transform:
+  // held through the whole transform its 64 registers would spill.
+  constexpr bool E_EARLY = sizeof(T) == 4;
+  T e[KT];
+  auto load_e = [&](const T* ea) {
+#pragma unroll
+    for (int k = 0; k < KT; ++k) e[k] = ea[gidx(tid + NT * k)];
+  };
+  {
+    T v[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] = in_b[gidx(tid + NT * k)];
+    if (!fin) load_e(enext_b);
+    else if (E_EARLY) load_e(efin_b);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) pl[lds_at(tid + NT * k)] = fin ? v[k] : v[k] * e[k];
+  }
+  __syncthreads();
+
+  if (fin) {
+    // finish the previous step along the row axis
+    slab_transform<T, NT, false, KIND>(pl, tab, tab + 3 * M, p.tab_r + 2 * M, [&] {
+      if (!E_EARLY) load_e(efin_b);
+    });
+    // q_{s+1} = E_s (.) -> history (non-temporal); times E_{s+1}
+    const bool junction = next && ch.e_next != ch.e_fin;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int f = tid + NT * k;
+      const int a = lds_at(f);
+      const int g = gidx(f);
+      T q = pl[a] * e[k];
+      __builtin_nontemporal_store(q, &hist_b[g]);
+      if (next) pl[a] = q * (junction ? enext_b[g] : e[k]);
+    }
+    if (!next) return;
+    __syncthreads();
+  }
+  // the next step: z, then the row axis
+  slab_transform<T, NT, true, KIND>(pl, tab, tab + 2 * M, p.tab_c + 2 * M);
+  slab_transform<T, NT, false, KIND>(pl, tab, tab + 3 * M, p.tab_r + 2 * M);
+  // the half-finished plane for the next phase (non-temporal: read from
+  // other XCDs)
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int f = tid + NT * k;
+    __builtin_nontemporal_store(pl[lds_at(f)], &out_b[gidx(f)]);
+  }
+}
+
+template <typename T>
+hipError_t launch_radix_phase(const PhaseParams<T>& p, int nplanes, int nchains, hipStream_t s) {
+  if (nchains < 1 || nchains > 2 || p.MR != 128 || p.MC != 128 || !p.tab_r || !p.tab_c ||
+      p.row_stride * 127 + 128 > 0x7fffffffL)
+    return hipErrorInvalidValue;
+  dim3 grid((unsigned)nplanes, (unsigned)nchains, 1);
+  if (p.rkind == kRadixP1)
+    hipLaunchKernelGGL((radix_phase_kernel<T, kRadixP1>), grid, dim3(512), 0, s, p);
+  else if (p.rkind == kRadixDct)
+    hipLaunchKernelGGL((radix_phase_kernel<T, kRadixDct>), grid, dim3(512), 0, s, p);
+  else
+    hipLaunchKernelGGL((radix_phase_kernel<T, kRadixM3>), grid, dim3(512), 0, s, p);
   return hipGetLastError();
 }
 
