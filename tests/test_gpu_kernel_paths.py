@@ -1,12 +1,14 @@
 """Every kernel path of the SCFT iteration against the oracle (needs an MI355X).
 
 The iteration picks its contour-step kernels by grid: the one-launch plane
-phases (split over two workgroups per plane at class 64, or one), the fused
-z+y slab plus the x pass, or three axis passes; the centrosymmetric products
-where they apply.  The environment switches CRYS_PHASE, CRYS_PHASE_HALF,
-CRYS_SLAB and CRYS_CENTRO force the other paths; each is read once per
-process, so each case runs the GPU side in a child process.  FP64 traces must
-stay within 1e-10 of the oracle on every path.
+phases (split over two workgroups per plane at class 64, or one; the radix
+plane phase when every axis has length 128), the fused z+y slab (matrix-core,
+or radix at length 128) plus the x pass, or three axis passes; the
+centrosymmetric products where they apply.  The environment switches
+CRYS_PHASE, CRYS_PHASE_HALF, CRYS_SLAB, CRYS_CENTRO and CRYS_AXIS_PASS force
+the other paths; each is read once per process, so each case runs the GPU
+side in a child process.  FP64 traces must stay within 1e-10 of the oracle on
+every path.
 """
 import json
 import os
@@ -37,35 +39,39 @@ run.set_fields(F.extract(wA, name), F.extract(wB, name))
 print(json.dumps(run.run(nstep).tolist()))
 """
 
-CASES = [("PmmmDCT", (128, 128, 128), (4.2,) * 3), ("P1", (48, 48, 48), (4.2,) * 3)]
+# (solver, logical grid, cell, ds): length classes 64 and 48, and length 128
+# on every axis (radix phase / radix slab / radix passes / matrix-core passes)
+CASES = [("PmmmDCT", (128, 128, 128), (4.2,) * 3, 0.01), ("P1", (48, 48, 48), (4.2,) * 3, 0.01),
+         ("PmmmDCT", (256, 256, 256), (5.0,) * 3, 0.05)]
 SWITCHES = [{}, {"CRYS_PHASE_HALF": "0"}, {"CRYS_PHASE": "0"},
-            {"CRYS_PHASE": "0", "CRYS_SLAB": "0"}, {"CRYS_CENTRO": "0"}]
-FA, XN, DS, ACC, NSTEP = 0.3, 25.0, 0.01, 0.1, 2
+            {"CRYS_PHASE": "0", "CRYS_SLAB": "0"}, {"CRYS_CENTRO": "0"},
+            {"CRYS_AXIS_PASS": "mfma"}]
+FA, XN, ACC, NSTEP = 0.3, 25.0, 0.1, 2
 
 
 @pytest.fixture(scope="module")
 def references(oracle):
     out = {}
-    for name, N, cell in CASES:
+    for name, N, cell, ds in CASES:
         wA, wB = F.bcc_fields(N, cell, FA, XN)
         ref = oracle.Solver(name, N)
-        tr, *_ = ref.scft(FA, DS, XN, cell, ACC, NSTEP, F.extract(wA, name), F.extract(wB, name))
-        out[name] = tr
+        tr, *_ = ref.scft(FA, ds, XN, cell, ACC, NSTEP, F.extract(wA, name), F.extract(wB, name))
+        out[(name, N)] = tr
     return out
 
 
 @pytest.mark.parametrize("switch", SWITCHES, ids=lambda d: ",".join(f"{k}={v}" for k, v in d.items()) or "default")
-@pytest.mark.parametrize("name,N,cell", CASES)
-def test_scft_path_matches_oracle(references, switch, name, N, cell):
+@pytest.mark.parametrize("name,N,cell,ds", CASES)
+def test_scft_path_matches_oracle(references, switch, name, N, cell, ds):
     env = dict(os.environ)
     env.update(switch)
-    arg = json.dumps([name, list(N), list(cell), FA, XN, DS, ACC, NSTEP])
+    arg = json.dumps([name, list(N), list(cell), FA, XN, ds, ACC, NSTEP])
     print("kernel path", name, N, switch, flush=True, file=sys.stderr)
     r = subprocess.run([sys.executable, "-c", CHILD.format(root=ROOT), arg], env=env,
                        capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr[-2000:]
     tr = np.array(json.loads(r.stdout.strip().splitlines()[-1]))
-    ref = references[name]
+    ref = references[(name, N)]
     dF = np.abs(tr[:, 0] / ref[:, 0] - 1).max()
     dM = np.abs(tr[:, 5] / ref[:, 5] - 1).max()
     print(f"  |dF/F| {dF:.2e}  |dMaxInc| {dM:.2e}", flush=True, file=sys.stderr)
