@@ -284,16 +284,17 @@ print(json.dumps({{"iter_s": iters / dt, "ms_step_repeated": ms_step, "ms_per_la
 PASS_GRIDS = [("PmmmDCT", 128, 4.2), ("PmmmDCT", 256, 4.2), ("P1", 128, 4.2)]
 PASS_PATHS = {"phase": {}, "slab+x": {"CRYS_PHASE": "0"},
               "mfma_passes": {"CRYS_PHASE": "0", "CRYS_SLAB": "0", "CRYS_AXIS_PASS": "mfma"},
-              "radix_passes": {"CRYS_PHASE": "0", "CRYS_AXIS_PASS": "radix"}}
+              "radix_passes": {"CRYS_PHASE": "0", "CRYS_SLAB": "0", "CRYS_AXIS_PASS": "radix"}}
 
 
 def passes_timing(iters, only=None):
     """One JSON line: ms per contour step (both chains) of each kernel path
-    of the SCFT iteration, on the headline grid and on two grids of axis
-    length 128: the one-launch plane phase, the fused z+y slab plus the x
-    pass, three matrix-core passes (dense operators as GEMMs) and three
-    radix FFT/DCT passes.  Each path runs in a child process because the
-    path switches are read once per process."""
+    of the SCFT iteration, on the headline grid (axis length 64) and on two
+    grids of axis length 128: the one-launch plane phase (matrix-core at 64,
+    radix at 128), the fused z+y slab plus the x pass (likewise), three
+    matrix-core passes (dense operators as GEMMs) and three radix FFT/DCT
+    passes.  Each path runs in a child process because the path switches are
+    read once per process."""
     import subprocess
 
     out = {"metric": "ms per contour step (both chains) by kernel path, SCFT iteration, Ns=200",
@@ -306,10 +307,6 @@ def passes_timing(iters, only=None):
         for prec in ("fp64", "fp32"):
             b = 8 if prec == "fp64" else 4
             for path, env_add in PASS_PATHS.items():
-                if N > 128 and name != "P1" and path in ("phase", "slab+x"):
-                    continue  # length class 128: the passes are the only path
-                if name == "P1" and path in ("phase", "slab+x"):
-                    continue
                 env = dict(os.environ, **env_add)
                 arg = json.dumps([name, [N] * 3, L, prec, iters])
                 print(f"bench: passes {name} {N}^3 {prec} {path}", file=sys.stderr, flush=True)
