@@ -429,6 +429,110 @@ __device__ __forceinline__ void slab_stage(T* pl, const T* tw, bool twiddle) {
   (void)M;
 }
 
+// The spectral pair (Z_k, Z_{M-k}) -> (W_k, W_{M-k}) of the DCT kinds, in
+// place (see the pass kernel's spectral step).
+template <typename T, int M>
+__device__ __forceinline__ void dct_spec_pair(T& zr, T& zi, T& mr, T& mi, int k, const T* kp,
+                                              const T* cq) {
+  const T yr = mr, yi = -mi;                    // conj(Z_{M-k})
+  const T cr = cq[2 * k], ci = -cq[2 * k + 1];  // c_k = exp(-i pi k / 2M)
+  const T K0 = kp[k], K1 = kp[(M - k) & (M - 1)];
+  const T sr = zr + yr, si = zi + yi, dr = zr - yr, di = zi - yi;
+  const T par = cr * sr - ci * si, pai = cr * si + ci * sr;
+  const T qr = cr * dr - ci * di, qi = cr * di + ci * dr;
+  const T pbr = qi, pbi = -qr;
+  const T ar = K0 * par, ai = K1 * pai, br = K0 * pbr, bi = K1 * pbi;
+  const T war = cr * ar + ci * ai, wai = cr * ai - ci * ar;
+  const T wbr = cr * br + ci * bi, wbi = cr * bi - ci * br;
+  zr = war - wbi;
+  zi = wai + wbr;
+  mr = war + wbi;
+  mi = wbr - wai;
+}
+
+// Forward stage B, the spectral step and inverse stage A' in one pass over
+// the registers (slab and phase kernels, M = 128): forward B of residue s
+// leaves Z_(s + R1 k2) at the positions s R2 + k2 that inverse A' of residue
+// s reads back, and the spectral partner of residue s is residue R1 - s.  Job
+// (c, u), u < R1 / 2, holds residues {u, R1 - u} ({0, R1 / 2} for u = 0), so
+// every pair is in one job's registers and the two LDS round trips and
+// barriers between the three stages go away.
+template <typename T, int NT, bool Z_DIR, int KIND>
+__device__ __forceinline__ void slab_fused_bsa(T* pl, const T* tw, const T* kp, const T* cq) {
+  using S = RadixSlabShape<T>;
+  constexpr int M = S::M, R1 = S::R1, R2 = S::R2, NCL = S::NCL, RP = S::RP, U = R1 / 2;
+  for (int job = threadIdx.x; job < NCL * U; job += NT) {
+    const int c = Z_DIR ? job / U : job % NCL;
+    const int u = Z_DIR ? job - c * U : job / NCL;
+    const int sa = u, sb = u == 0 ? R1 / 2 : R1 - u;
+    auto at = [&](int pos, int part) {
+      return Z_DIR ? (2 * c + part) * RP + S::col(pos) : pos * RP + S::col(2 * c + part);
+    };
+    T ar[R2], ai[R2], br[R2], bi[R2];
+#pragma unroll
+    for (int t = 0; t < R2; ++t) {
+      ar[t] = pl[at(sa * R2 + t, 0)];
+      ai[t] = pl[at(sa * R2 + t, 1)];
+      br[t] = pl[at(sb * R2 + t, 0)];
+      bi[t] = pl[at(sb * R2 + t, 1)];
+    }
+    dft_reg<T, R2, -1>(ar, ai);  // Z_(sa + R1 k2)
+    dft_reg<T, R2, -1>(br, bi);  // Z_(sb + R1 k2)
+    if constexpr (KIND == kRadixP1) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        const T ka = kp[sa + R1 * k2], kb = kp[sb + R1 * k2];
+        ar[k2] *= ka;
+        ai[k2] *= ka;
+        br[k2] *= kb;
+        bi[k2] *= kb;
+      }
+    } else if (u == 0) {
+      // residue 0: k = 0 and M / 2 pair with themselves, k2 with R2 - k2
+#pragma unroll
+      for (int k2 = 0; k2 <= R2 / 2; k2 += R2 / 2) {
+        T mr = ar[k2], mi = ai[k2];
+        dct_spec_pair<T, M>(ar[k2], ai[k2], mr, mi, R1 * k2, kp, cq);
+      }
+#pragma unroll
+      for (int k2 = 1; k2 < R2 / 2; ++k2)
+        dct_spec_pair<T, M>(ar[k2], ai[k2], ar[R2 - k2], ai[R2 - k2], R1 * k2, kp, cq);
+      // residue R1 / 2: k2 with R2 - 1 - k2
+#pragma unroll
+      for (int k2 = 0; k2 < R2 / 2; ++k2)
+        dct_spec_pair<T, M>(br[k2], bi[k2], br[R2 - 1 - k2], bi[R2 - 1 - k2], R1 / 2 + R1 * k2, kp,
+                            cq);
+    } else {
+      // residue sa with residue sb = R1 - sa: k2 with R2 - 1 - k2
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2)
+        dct_spec_pair<T, M>(ar[k2], ai[k2], br[R2 - 1 - k2], bi[R2 - 1 - k2], sa + R1 * k2, kp,
+                            cq);
+    }
+    // inverse A': DFT over k2, twiddle w^(j2 s), back to positions s R2 + j2
+    dft_reg<T, R2, 1>(ar, ai);
+    dft_reg<T, R2, 1>(br, bi);
+#pragma unroll
+    for (int j2 = 1; j2 < R2; ++j2) {
+      const int ea = j2 * sa, eb = j2 * sb;  // < M
+      const T ca = tw[2 * ea], sna = tw[2 * ea + 1], cb = tw[2 * eb], snb = tw[2 * eb + 1];
+      const T xa = ar[j2] * ca - ai[j2] * sna;
+      ai[j2] = ar[j2] * sna + ai[j2] * ca;
+      ar[j2] = xa;
+      const T xb = br[j2] * cb - bi[j2] * snb;
+      bi[j2] = br[j2] * snb + bi[j2] * cb;
+      br[j2] = xb;
+    }
+#pragma unroll
+    for (int t = 0; t < R2; ++t) {
+      pl[at(sa * R2 + t, 0)] = ar[t];
+      pl[at(sa * R2 + t, 1)] = ai[t];
+      pl[at(sb * R2 + t, 0)] = br[t];
+      pl[at(sb * R2 + t, 1)] = bi[t];
+    }
+  }
+}
+
 // Forward FFT, spectral step (K' of this direction; cq the DCT twiddles in
 // global memory) and inverse FFT of every packed line of one direction.
 // NT: the threads of the calling workgroup; `before_last()` runs before the
@@ -437,49 +541,10 @@ template <typename T, int NT, bool Z_DIR, int KIND, typename F>
 __device__ __forceinline__ void slab_transform(T* pl, const T* tw, const T* kp, const T* cq,
                                                F before_last) {
   using S = RadixSlabShape<T>;
-  constexpr int M = S::M, R1 = S::R1, R2 = S::R2, NCL = S::NCL, RP = S::RP;
+  constexpr int R1 = S::R1, R2 = S::R2;
   slab_stage<T, NT, Z_DIR, R1, R2, -1, R2, 1>(pl, tw, true);   // A: points t + R2 m
   __syncthreads();
-  slab_stage<T, NT, Z_DIR, R2, R1, -1, 1, R2>(pl, tw, false);  // B: points s R2 + t
-  __syncthreads();
-  // Z_k sits at position (k % R1) R2 + k / R1
-  auto at = [&](int c, int k, int part) {
-    const int pos = (k % R1) * R2 + k / R1;
-    return Z_DIR ? (2 * c + part) * RP + S::col(pos) : pos * RP + S::col(2 * c + part);
-  };
-  if constexpr (KIND == kRadixP1) {
-    for (int f = threadIdx.x; f < NCL * M; f += NT) {
-      const int c = Z_DIR ? f / M : f % NCL, k = Z_DIR ? f - c * M : f / NCL;
-      pl[at(c, k, 0)] *= kp[k];
-      pl[at(c, k, 1)] *= kp[k];
-    }
-  } else {
-    constexpr int HP = M / 2 + 1;
-    for (int f = threadIdx.x; f < NCL * HP; f += NT) {
-      const int c = Z_DIR ? f / HP : f % NCL, k = Z_DIR ? f - c * HP : f / NCL;
-      const int km = (M - k) & (M - 1);
-      const int a0 = at(c, k, 0), a1 = at(c, k, 1), b0 = at(c, km, 0), b1 = at(c, km, 1);
-      const T zr = pl[a0], zi = pl[a1];
-      const T yr = pl[b0], yi = -pl[b1];            // conj(Z_{M-k})
-      const T cr = cq[2 * k], ci = -cq[2 * k + 1];  // c_k = exp(-i pi k / 2M)
-      const T K0 = kp[k], K1 = kp[km];
-      const T sr = zr + yr, si = zi + yi, dr = zr - yr, di = zi - yi;
-      const T par = cr * sr - ci * si, pai = cr * si + ci * sr;
-      const T qr = cr * dr - ci * di, qi = cr * di + ci * dr;
-      const T pbr = qi, pbi = -qr;
-      const T ar = K0 * par, ai = K1 * pai, br = K0 * pbr, bi = K1 * pbi;
-      const T war = cr * ar + ci * ai, wai = cr * ai - ci * ar;
-      const T wbr = cr * br + ci * bi, wbi = cr * bi - ci * br;
-      pl[a0] = war - wbi;
-      pl[a1] = wai + wbr;
-      if (km != k) {
-        pl[b0] = war + wbi;
-        pl[b1] = wbr - wai;
-      }
-    }
-  }
-  __syncthreads();
-  slab_stage<T, NT, Z_DIR, R2, R1, 1, 1, R2>(pl, tw, true);    // A': positions s R2 + k2
+  slab_fused_bsa<T, NT, Z_DIR, KIND>(pl, tw, kp, cq);          // B, spectral, A'
   __syncthreads();
   before_last();
   slab_stage<T, NT, Z_DIR, R1, R2, 1, R2, 1>(pl, tw, false);   // B': positions s R2 + j2
