@@ -32,13 +32,18 @@
 //
 //   forward  A: job (c, t), t < R2: DFT_R1 over the points t + R2 m, times
 //               w^(t k1), back to positions t + R2 k1;
-//   forward  B: job (c, s), s < R1: DFT_R2 over the positions s R2 + t; the
+//   forward  B: residue s < R1: DFT_R2 over the positions s R2 + t; the
 //               output Z_(s + R1 k2) stays at position s R2 + k2;
-//   spectral:   the pairs (k, M-k) in that order;
-//   inverse  A: job (c, s): DFT_R2 over the positions s R2 + k2, times
+//   spectral:   the pairs (k, M-k);
+//   inverse  A: residue s: DFT_R2 over the positions s R2 + k2, times
 //               w^(j2 s), back to positions s R2 + j2;
 //   inverse  B: job (c, j2), j2 < R2: DFT_R1 over the positions s R2 + j2;
 //               output j1 goes to position j1 R2 + j2 = j, the natural order.
+//
+// Forward B, the spectral step and inverse A run as one stage in registers:
+// the partner of Z_k (residue s = k mod R1) is Z_(M-k), residue R1 - s, so a
+// job that holds residues s and R1 - s (0 and R1 / 2 together) has every pair
+// it needs.
 //
 // Position p sits at p + p / R2 in its plane, so the R1 or R2 lanes of one
 // job group and the lanes of neighbouring jobs fall on distinct banks in
@@ -193,6 +198,101 @@ __device__ __forceinline__ int radix_pos(int i) {
   else return i;
 }
 
+// The spectral pair (Z_k, Z_{M-k}) -> (W_k, W_{M-k}) of the DCT kinds, in
+// place (see the pass kernel's spectral step).
+template <typename T, int M>
+__device__ __forceinline__ void dct_spec_pair(T& zr, T& zi, T& mr, T& mi, int k, const T* kp,
+                                              const T* cq) {
+  const T yr = mr, yi = -mi;                    // conj(Z_{M-k})
+  const T cr = cq[2 * k], ci = -cq[2 * k + 1];  // c_k = exp(-i pi k / 2M)
+  const T K0 = kp[k], K1 = kp[(M - k) & (M - 1)];
+  const T sr = zr + yr, si = zi + yi, dr = zr - yr, di = zi - yi;
+  const T par = cr * sr - ci * si, pai = cr * si + ci * sr;
+  const T qr = cr * dr - ci * di, qi = cr * di + ci * dr;
+  const T pbr = qi, pbi = -qr;
+  const T ar = K0 * par, ai = K1 * pai, br = K0 * pbr, bi = K1 * pbi;
+  const T war = cr * ar + ci * ai, wai = cr * ai - ci * ar;
+  const T wbr = cr * br + ci * bi, wbi = cr * bi - ci * br;
+  zr = war - wbi;
+  zi = wai + wbr;
+  mr = war + wbi;
+  mi = wbr - wai;
+}
+
+// Forward stage B, the spectral step and inverse stage A' of the pass kernel
+// in one pass over the registers (see slab_fused_bsa below for the pairing of
+// residues s and R1 - s): job (c, u), u < R1 / 2, on the planes of packed
+// line c.
+template <typename T, int M, int KIND>
+__device__ __forceinline__ void radix_fused_bsa(T* buf, const T* tw, const T* kp, const T* cq) {
+  using S = RadixShape<M>;
+  constexpr int R1 = S::R1, R2 = S::R2, NC = S::NC, PP = S::PP, U = R1 / 2;
+  for (int job = threadIdx.x; job < NC * U; job += S::NT) {
+    const int c = job / U, u = job - c * U;
+    const int sa = u, sb = u == 0 ? R1 / 2 : R1 - u;
+    T* const pr = buf + c * PP;
+    T* const pi = buf + (NC + c) * PP;
+    T ar[R2], ai[R2], br[R2], bi[R2];
+#pragma unroll
+    for (int t = 0; t < R2; ++t) {
+      ar[t] = pr[S::phys(sa * R2 + t)];
+      ai[t] = pi[S::phys(sa * R2 + t)];
+      br[t] = pr[S::phys(sb * R2 + t)];
+      bi[t] = pi[S::phys(sb * R2 + t)];
+    }
+    dft_reg<T, R2, -1>(ar, ai);  // Z_(sa + R1 k2)
+    dft_reg<T, R2, -1>(br, bi);  // Z_(sb + R1 k2)
+    if constexpr (KIND == kRadixP1) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        const T ka = kp[sa + R1 * k2], kb = kp[sb + R1 * k2];
+        ar[k2] *= ka;
+        ai[k2] *= ka;
+        br[k2] *= kb;
+        bi[k2] *= kb;
+      }
+    } else if (u == 0) {
+#pragma unroll
+      for (int k2 = 0; k2 <= R2 / 2; k2 += R2 / 2) {
+        T mr = ar[k2], mi = ai[k2];
+        dct_spec_pair<T, M>(ar[k2], ai[k2], mr, mi, R1 * k2, kp, cq);
+      }
+#pragma unroll
+      for (int k2 = 1; k2 < R2 / 2; ++k2)
+        dct_spec_pair<T, M>(ar[k2], ai[k2], ar[R2 - k2], ai[R2 - k2], R1 * k2, kp, cq);
+#pragma unroll
+      for (int k2 = 0; k2 < R2 / 2; ++k2)
+        dct_spec_pair<T, M>(br[k2], bi[k2], br[R2 - 1 - k2], bi[R2 - 1 - k2], R1 / 2 + R1 * k2, kp,
+                            cq);
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2)
+        dct_spec_pair<T, M>(ar[k2], ai[k2], br[R2 - 1 - k2], bi[R2 - 1 - k2], sa + R1 * k2, kp,
+                            cq);
+    }
+    dft_reg<T, R2, 1>(ar, ai);
+    dft_reg<T, R2, 1>(br, bi);
+#pragma unroll
+    for (int j2 = 1; j2 < R2; ++j2) {
+      const int ea = j2 * sa, eb = j2 * sb;  // < M
+      const T ca = tw[2 * ea], sna = tw[2 * ea + 1], cb = tw[2 * eb], snb = tw[2 * eb + 1];
+      const T xa = ar[j2] * ca - ai[j2] * sna;
+      ai[j2] = ar[j2] * sna + ai[j2] * ca;
+      ar[j2] = xa;
+      const T xb = br[j2] * cb - bi[j2] * snb;
+      bi[j2] = br[j2] * snb + bi[j2] * cb;
+      br[j2] = xb;
+    }
+#pragma unroll
+    for (int t = 0; t < R2; ++t) {
+      pr[S::phys(sa * R2 + t)] = ar[t];
+      pi[S::phys(sa * R2 + t)] = ai[t];
+      pr[S::phys(sb * R2 + t)] = br[t];
+      pi[S::phys(sb * R2 + t)] = bi[t];
+    }
+  }
+}
+
 // Waves per SIMD the compiler must allow: 4 (four 256-thread workgroups per
 // CU) where that fits in 128 VGPRs without spilling; the FP64 length-128
 // kernel needs about 150 and measured slower when held to 128 (spills).
@@ -283,53 +383,61 @@ __attribute__((amdgpu_waves_per_eu(radix_waves_per_eu<T, M>()))) void radix_pass
   // ---- forward FFT of z = line 2c + i line 2c+1 ----
   radix_stage<T, M, R1, R2, -1, R2, 1>(buf, tw, true);   // A: points t + R2 m
   __syncthreads();
-  radix_stage<T, M, R2, R1, -1, 1, R2>(buf, tw, false);  // B: points s R2 + t
-  __syncthreads();
-
-  // ---- spectral step; Z_k sits at position (k % R1) R2 + k / R1 ----
-  auto zpos = [](int k) { return S::phys((k % R1) * R2 + k / R1); };
-  if constexpr (KIND == kRadixP1) {
-    for (int f = tid; f < NC * M; f += NT) {
-      const int c = f / M, k = f - c * M;
-      const int q = zpos(k);
-      buf[c * PP + q] *= kp[k];
-      buf[(NC + c) * PP + q] *= kp[k];
-    }
+  if constexpr (sizeof(T) == 4 && M == 128) {
+    // B, spectral and A' in registers: measured faster only here (FP32 at
+    // length 128); elsewhere its R1 / 2 jobs per line leave too many threads
+    // idle
+    radix_fused_bsa<T, M, KIND>(buf, tw, kp, cq);
+    __syncthreads();
   } else {
-    constexpr int HP = M / 2 + 1;  // pairs (k, M-k), k = 0 .. M/2
-    for (int f = tid; f < NC * HP; f += NT) {
-      const int c = f / HP, k = f - c * HP;
-      const int km = (M - k) & (M - 1);
-      T* const pr = buf + c * PP;
-      T* const pi = buf + (NC + c) * PP;
-      const int q0 = zpos(k), q1 = zpos(km);
-      const T zr = pr[q0], zi = pi[q0];
-      const T yr = pr[q1], yi = -pi[q1];           // conj(Z_{M-k})
-      const T cr = cq[2 * k], ci = -cq[2 * k + 1];  // c_k = exp(-i pi k / 2M)
-      const T K0 = kp[k], K1 = kp[km];
-      // line a: P = c_k (Z_k + conj Z_{M-k}); line b: P = -i c_k (Z_k - conj Z_{M-k})
-      const T sr = zr + yr, si = zi + yi, dr = zr - yr, di = zi - yi;
-      const T par = cr * sr - ci * si, pai = cr * si + ci * sr;
-      const T qr = cr * dr - ci * di, qi = cr * di + ci * dr;
-      const T pbr = qi, pbi = -qr;
-      // W = conj(c_k) (K'_k Re P + i K'_{M-k} Im P)
-      const T ar = K0 * par, ai = K1 * pai, br = K0 * pbr, bi = K1 * pbi;
-      const T war = cr * ar + ci * ai, wai = cr * ai - ci * ar;
-      const T wbr = cr * br + ci * bi, wbi = cr * bi - ci * br;
-      // packed: W_k = Wa + i Wb, W_{M-k} = conj(Wa) + i conj(Wb)
-      pr[q0] = war - wbi;
-      pi[q0] = wai + wbr;
-      if (km != k) {
-        pr[q1] = war + wbi;
-        pi[q1] = wbr - wai;
+    radix_stage<T, M, R2, R1, -1, 1, R2>(buf, tw, false);  // B: points s R2 + t
+    __syncthreads();
+
+    // ---- spectral step; Z_k sits at position (k % R1) R2 + k / R1 ----
+    auto zpos = [](int k) { return S::phys((k % R1) * R2 + k / R1); };
+    if constexpr (KIND == kRadixP1) {
+      for (int f = tid; f < NC * M; f += NT) {
+        const int c = f / M, k = f - c * M;
+        const int q = zpos(k);
+        buf[c * PP + q] *= kp[k];
+        buf[(NC + c) * PP + q] *= kp[k];
+      }
+    } else {
+      constexpr int HP = M / 2 + 1;  // pairs (k, M-k), k = 0 .. M/2
+      for (int f = tid; f < NC * HP; f += NT) {
+        const int c = f / HP, k = f - c * HP;
+        const int km = (M - k) & (M - 1);
+        T* const pr = buf + c * PP;
+        T* const pi = buf + (NC + c) * PP;
+        const int q0 = zpos(k), q1 = zpos(km);
+        const T zr = pr[q0], zi = pi[q0];
+        const T yr = pr[q1], yi = -pi[q1];           // conj(Z_{M-k})
+        const T cr = cq[2 * k], ci = -cq[2 * k + 1];  // c_k = exp(-i pi k / 2M)
+        const T K0 = kp[k], K1 = kp[km];
+        // line a: P = c_k (Z_k + conj Z_{M-k}); line b: P = -i c_k (Z_k - conj Z_{M-k})
+        const T sr = zr + yr, si = zi + yi, dr = zr - yr, di = zi - yi;
+        const T par = cr * sr - ci * si, pai = cr * si + ci * sr;
+        const T qr = cr * dr - ci * di, qi = cr * di + ci * dr;
+        const T pbr = qi, pbi = -qr;
+        // W = conj(c_k) (K'_k Re P + i K'_{M-k} Im P)
+        const T ar = K0 * par, ai = K1 * pai, br = K0 * pbr, bi = K1 * pbi;
+        const T war = cr * ar + ci * ai, wai = cr * ai - ci * ar;
+        const T wbr = cr * br + ci * bi, wbi = cr * bi - ci * br;
+        // packed: W_k = Wa + i Wb, W_{M-k} = conj(Wa) + i conj(Wb)
+        pr[q0] = war - wbi;
+        pi[q0] = wai + wbr;
+        if (km != k) {
+          pr[q1] = war + wbi;
+          pi[q1] = wbr - wai;
+        }
       }
     }
-  }
-  __syncthreads();
+    __syncthreads();
 
-  // ---- inverse FFT back to the natural order; Re / Im are the two lines ----
-  radix_stage<T, M, R2, R1, 1, 1, R2>(buf, tw, true);    // A': positions s R2 + k2
-  __syncthreads();
+    // ---- inverse FFT back to the natural order; Re / Im are the two lines ----
+    radix_stage<T, M, R2, R1, 1, 1, R2>(buf, tw, true);    // A': positions s R2 + k2
+    __syncthreads();
+  }
   radix_stage<T, M, R1, R2, 1, R2, 1>(buf, tw, false);   // B': positions s R2 + j2
   __syncthreads();
 
@@ -427,27 +535,6 @@ __device__ __forceinline__ void slab_stage(T* pl, const T* tw, bool twiddle) {
     }
   }
   (void)M;
-}
-
-// The spectral pair (Z_k, Z_{M-k}) -> (W_k, W_{M-k}) of the DCT kinds, in
-// place (see the pass kernel's spectral step).
-template <typename T, int M>
-__device__ __forceinline__ void dct_spec_pair(T& zr, T& zi, T& mr, T& mi, int k, const T* kp,
-                                              const T* cq) {
-  const T yr = mr, yi = -mi;                    // conj(Z_{M-k})
-  const T cr = cq[2 * k], ci = -cq[2 * k + 1];  // c_k = exp(-i pi k / 2M)
-  const T K0 = kp[k], K1 = kp[(M - k) & (M - 1)];
-  const T sr = zr + yr, si = zi + yi, dr = zr - yr, di = zi - yi;
-  const T par = cr * sr - ci * si, pai = cr * si + ci * sr;
-  const T qr = cr * dr - ci * di, qi = cr * di + ci * dr;
-  const T pbr = qi, pbi = -qr;
-  const T ar = K0 * par, ai = K1 * pai, br = K0 * pbr, bi = K1 * pbi;
-  const T war = cr * ar + ci * ai, wai = cr * ai - ci * ar;
-  const T wbr = cr * br + ci * bi, wbi = cr * bi - ci * br;
-  zr = war - wbi;
-  zi = wai + wbr;
-  mr = war + wbi;
-  mi = wbr - wai;
 }
 
 // Forward stage B, the spectral step and inverse stage A' in one pass over
