@@ -353,7 +353,10 @@ def test_non_finite_fields_report_divergence():
 # passes of 512 threads on length 96, dense class 128) next to 64.
 LARGE = [("P1", (128, 96, 64), (4.4, 4.1, 3.9), 0.005),
          ("PmmmDCT", (256, 192, 128), (5.0, 5.0, 5.0), 0.005),
-         ("3m", (256, 192, 128), (5.0, 4.6, 4.2), 0.005)]
+         ("3m", (256, 192, 128), (5.0, 4.6, 4.2), 0.005),
+         # y and z of length 128: the fused z+y radix slab, then a length-64 x pass
+         ("PmmmDCT", (128, 256, 256), (4.0, 5.0, 5.0), 0.005),
+         ("P1", (64, 128, 128), (4.0, 4.4, 4.6), 0.005)]
 
 
 @pytest.mark.parametrize("name,N,cell,ds", LARGE)
@@ -398,5 +401,23 @@ def test_scft_class_128(oracle, name, N, cell, prec):
     tol = 1e-10 if prec == "fp64" else 1e-5
     assert dF <= tol and dM <= (tol if prec == "fp64" else 1e-3)
     assert rel_err(run.fields()[0], phiA_ref) < (1e-10 if prec == "fp64" else 1e-5)
+    run.close()
+    s.close()
+
+
+def test_scft_slab_radix_with_x_pass(oracle):
+    """y and z of length 128, x of length 64: the SCFT iteration takes the
+    radix slab plus a matrix-core x pass per contour step."""
+    N, cell, fA, xN, ds, acc, nstep = (128, 256, 256), (4.0, 5.0, 5.0), 0.3, 20.0, 0.05, 0.1, 2
+    wA, wB = F.bcc_fields(N, cell, fA, xN)
+    a, b = F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT")
+    tr_ref, *_ = oracle.Solver("PmmmDCT", N).scft(fA, ds, xN, cell, acc, nstep, a, b)
+    s = C.SolverPmmmDCT(N)
+    run = C.SCFT(s, fA, ds, xN, cell, acc)
+    run.set_fields(a, b)
+    tr = run.run(nstep)
+    dF = np.abs(tr[:, 0] / tr_ref[:, 0] - 1).max()
+    say(f"  128x256x256 Pmmm: |dF/F| {dF:.2e}")
+    assert dF <= 1e-10
     run.close()
     s.close()
