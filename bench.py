@@ -35,6 +35,8 @@ sys.path.insert(0, ROOT)
 N_LOG = 128
 FA, XN, L, DS, ACC = 0.30, 25.0, 4.2, 1.0 / 200, 0.1
 HBM_PEAK_GBS = 8000.0
+# MI355X dense matrix-core peaks (FP64 and FP32 MFMA)
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
 # Kernels of one contour step in issue order, by the number of launches: the
 # one-launch plane phase (all axes in one length class <= 64), the fused z+y
@@ -397,6 +399,20 @@ def main():
                 "ms_per_launch": round(t_dom, 6),
                 "ms_per_launch_repeated_alone": round(t_alone, 6),
                 "bytes_per_launch": launch_bytes}
+    # Matrix-core use of the same kernel: every plane of both chains takes
+    # three half-size products per launch (M x M values, M / 2 multiply-adds
+    # each, M = 64); the split kernel also repeats the finishing product in
+    # the second workgroup of each plane, which is not counted.
+    # (per contour step, timed in the iteration's own sequence: one launch on
+    # the default path)
+    M = N_LOG // 2
+    flops_launch = 2 * M * 3 * (M * M * (M // 2) * 2)
+    t_step = f64["ms_per_step_in_sequence"]
+    mfma = {"achieved": round(flops_launch / (t_step * 1e-3) / 1e12, 3),
+            "peak": MFMA_PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+            "frac": round(flops_launch / (t_step * 1e-3) / 1e12 / MFMA_PEAK_TFLOPS["fp64"], 5),
+            "flops_per_step": flops_launch, "executed_flops_per_step": flops_launch * 4 // 3,
+            "kernel": roofline["kernel"]}
     step_bytes = 2 * 3 * n * 8  # minimum of a contour step: q_s, E, q_s+1 per chain
     step = {"ms": round(f64["ms_per_step_in_sequence"], 6),
             "ms_repeated_first_step": round(f64["ms_per_contour_step"], 6),
@@ -435,6 +451,7 @@ def main():
         "hbm_gbs": round(b_iter64 * f64["iter_s"] / 1e9, 2),
         "bytes_per_iter": b_iter64,
         "roofline": roofline,
+        "mfma": mfma,
         "contour_step": step,
         "cpu_baseline": cpu,
     }
@@ -450,6 +467,8 @@ def main():
                          "max_abs_dphiA_vs_fp64_gpu": float(np.abs(mixed["phiA"] - f64["phiA"]).max()),
                          "max_abs_dphiB_vs_fp64_gpu": float(np.abs(mixed["phiB"] - f64["phiB"]).max()),
                          "ms_per_contour_step": round(mixed["ms_per_step_in_sequence"], 6),
+                         "mfma_fp32_frac": round(flops_launch / (mixed["ms_per_step_in_sequence"] * 1e-3)
+                                                 / 1e12 / MFMA_PEAK_TFLOPS["fp32"], 5),
                          "ms_per_launch": {nm.replace("double", "float"): round(t, 6)
                                            for nm, t in zip(names, mixed["ms_per_launch"])}}
     print(json.dumps(line), flush=True)
