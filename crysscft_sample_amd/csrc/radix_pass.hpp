@@ -40,10 +40,12 @@
 //   inverse  B: job (c, j2), j2 < R2: DFT_R1 over the positions s R2 + j2;
 //               output j1 goes to position j1 R2 + j2 = j, the natural order.
 //
-// Forward B, the spectral step and inverse A run as one stage in registers:
-// the partner of Z_k (residue s = k mod R1) is Z_(M-k), residue R1 - s, so a
-// job that holds residues s and R1 - s (0 and R1 / 2 together) has every pair
-// it needs.
+// Forward B, the spectral step and inverse A can run as one stage in
+// registers: the partner of Z_k (residue s = k mod R1) is Z_(M-k), residue
+// R1 - s, so a job that holds residues s and R1 - s (0 and R1 / 2 together)
+// has every pair it needs.  The slab and phase kernels below always do this;
+// the pass kernel does it for FP32 at length 128 only, because elsewhere its
+// R1 / 2 jobs per line leave too many threads idle (measured slower).
 //
 // Position p sits at p + p / R2 in its plane, so the R1 or R2 lanes of one
 // job group and the lanes of neighbouring jobs fall on distinct banks in
