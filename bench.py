@@ -289,7 +289,7 @@ PASS_PATHS = {"phase": {}, "slab+x": {"CRYS_PHASE": "0"},
               "radix_passes": {"CRYS_PHASE": "0", "CRYS_SLAB": "0", "CRYS_AXIS_PASS": "radix"}}
 
 
-def passes_timing(iters, only=None):
+def passes_timing(iters, only=None, paths=None):
     """One JSON line: ms per contour step (both chains) of each kernel path
     of the SCFT iteration, on the headline grid (axis length 64) and on two
     grids of axis length 128: the one-launch plane phase (matrix-core at 64,
@@ -301,14 +301,21 @@ def passes_timing(iters, only=None):
 
     out = {"metric": "ms per contour step (both chains) by kernel path, SCFT iteration, Ns=200",
            "unit": "ms", "higher_is_better": False, "grids": {}}
-    for name, N, L in PASS_GRIDS:
-        if only and f"{name}:{N}" not in only:
-            continue
+    grids = PASS_GRIDS
+    if only:
+        # SOLVER:N or SOLVER:N:L (cell length, default 4.2)
+        grids = []
+        for g in only:
+            f = g.split(":")
+            grids.append((f[0], int(f[1]), float(f[2]) if len(f) > 2 else 4.2))
+    for name, N, L in grids:
         n = (N // 2) ** 3 if name != "P1" else N ** 3
         g = {}
         for prec in ("fp64", "fp32"):
             b = 8 if prec == "fp64" else 4
             for path, env_add in PASS_PATHS.items():
+                if paths and path not in paths:
+                    continue
                 env = dict(os.environ, **env_add)
                 arg = json.dumps([name, [N] * 3, L, prec, iters])
                 print(f"bench: passes {name} {N}^3 {prec} {path}", file=sys.stderr, flush=True)
@@ -342,8 +349,10 @@ def main():
     ap.add_argument("--passes", action="store_true",
                     help="time the contour step of every kernel path (phase, slab, matrix-core "
                          "passes, radix passes) and exit")
-    ap.add_argument("--pass-grid", action="append", metavar="SOLVER:N",
-                    help="with --passes: time only these grids (e.g. PmmmDCT:256)")
+    ap.add_argument("--pass-grid", action="append", metavar="SOLVER:N[:L]",
+                    help="with --passes: time these grids instead (e.g. PmmmDCT:256, 3m:64:4.0)")
+    ap.add_argument("--pass-path", action="append", choices=list(PASS_PATHS),
+                    help="with --passes: time only these kernel paths")
     ap.add_argument("--batch", type=int, default=0, metavar="COUNT",
                     help="time COUNT batched instances (the batching workload) and exit")
     args = ap.parse_args()
@@ -360,7 +369,7 @@ def main():
     if args.triplet:
         return triplet_timing(200)
     if args.passes:
-        return passes_timing(5, args.pass_grid)
+        return passes_timing(5, args.pass_grid, args.pass_path)
 
     from crysscft_sample_amd import fields as F
 
