@@ -41,7 +41,24 @@ struct PassParams {
   const unsigned short* mat_lo; // FP32 only: bf16 bits of At - (float)At, or nullptr
   int X, Y, Z;
   int cen;                      // 1: mat is the half-size centrosymmetric operator
+  int shift;                    // with cen: half-period pairing (see cen_partner)
 };
+
+// Pairing of the half-size (centrosymmetric) products.  A line is split into
+// u_i = x_i + x_{p(i)} and v_i = x_i - x_{p(i)} for i < M/2, where the
+// partner p(i) is the mirror M-1-i (PmmmDCT, P1: D[M-1-j][M-1-k] = D[j][k])
+// or, with `shift`, the half-period translation i + M/2 (3m with zero
+// translational parts: D[j+M/2][k+M/2] = D[j][k]).  The products run in the
+// paired basis, whose position M-1-i holds the element p(i); cen_index maps
+// a position of that basis to its index in memory.
+template <int MP>
+__device__ __forceinline__ int cen_partner(int i, bool shift) {
+  return shift ? i + MP / 2 : MP - 1 - i;
+}
+template <int MP>
+__device__ __forceinline__ int cen_index(int i, bool shift) {
+  return (shift && i >= MP / 2) ? (3 * MP / 2 - 1) - i : i;
+}
 
 // Register micro-tile width along the lines for a padded length MP.
 template <int MP>
@@ -276,12 +293,14 @@ struct MfmaShape {
 // least MP * MfmaShape<MP>::LDT elements.  Shared by the single-field kernel
 // and the batched kernel.  With CEN (centrosymmetric operator, M == MP,
 // MP % 32 == 0) the lines are split into u/v halves and `mat` is the
-// half-size operator At_c (M/2 x M) described at SlabShape.  The k groups of
-// every output tile are summed far-first (see SlabShape).
+// half-size operator At_c (M/2 x M) described at SlabShape, paired as
+// cen_partner(., shift) says.  The k groups of every output tile are summed
+// far-first (see SlabShape).
 template <typename T, int AXIS, int MP, bool CEN = false>
 __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* __restrict__ mat,
                                                const unsigned short* __restrict__ mat_lo, int X,
-                                               int Y, int Z, int tidx, T* tile) {
+                                               int Y, int Z, int tidx, T* tile,
+                                               bool shift = false) {
   using S = MfmaShape<MP>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int NT = S::NT;
@@ -377,14 +396,14 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
       x1[k] = ok[k] ? ch.in[gidx(si[k], sl[k])] : T(0);
-      x2[k] = ok[k] ? ch.in[gidx(MP - 1 - si[k], sl[k])] : T(0);
+      x2[k] = ok[k] ? ch.in[gidx(cen_partner<MP>(si[k], shift), sl[k])] : T(0);
     }
     if (ch.pre) {
       T e1[KT], e2[KT];
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
         e1[k] = ok[k] ? ch.pre[gidx(si[k], sl[k])] : T(0);
-        e2[k] = ok[k] ? ch.pre[gidx(MP - 1 - si[k], sl[k])] : T(0);
+        e2[k] = ok[k] ? ch.pre[gidx(cen_partner<MP>(si[k], shift), sl[k])] : T(0);
       }
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
@@ -473,7 +492,7 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
         e1[k] = ok[k] ? ch.post[gidx(si[k], sl[k])] : T(0);
-        e2[k] = ok[k] ? ch.post[gidx(MP - 1 - si[k], sl[k])] : T(0);
+        e2[k] = ok[k] ? ch.post[gidx(cen_partner<MP>(si[k], shift), sl[k])] : T(0);
       }
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
@@ -485,7 +504,7 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
     for (int k = 0; k < KT; ++k)
       if (ok[k]) {
         ch.out[gidx(si[k], sl[k])] = o1[k];
-        ch.out[gidx(MP - 1 - si[k], sl[k])] = o2[k];
+        ch.out[gidx(cen_partner<MP>(si[k], shift), sl[k])] = o2[k];
       }
   }
 }
@@ -494,7 +513,7 @@ template <typename T, int AXIS, int MP, bool CEN = false>
 __global__ __launch_bounds__(MfmaShape<MP>::NT) void axis_pass_mfma_kernel(PassParams<T> p) {
   __shared__ T tile[MP * MfmaShape<MP>::LDT];
   mfma_pass_tile<T, AXIS, MP, CEN>(p.c[blockIdx.y], p.mat, p.mat_lo, p.X, p.Y, p.Z, blockIdx.x,
-                                   tile);
+                                   tile, p.shift != 0);
 }
 
 // Padded length class for an axis of length M (0 if unsupported).
@@ -526,7 +545,8 @@ PassImpl pass_impl();
 // half-size blocks Ps = (D[j][k] + D[j][M-1-k])/2, Pa = (D[j][k] -
 // D[j][M-1-k])/2, and out_j = U_j + V_j, out_{M-1-j} = U_j - V_j.  This halves
 // the products.  The half-size operator is stored as At_c[k][j] with
-// k < M/2, j < M: columns j < M/2 hold Ps^T, columns M/2 + j hold Pa^T.
+// k < M/2, j < M: columns j < M/2 hold Ps^T, columns M/2 + j hold Pa^T.  For
+// 3m the partner of j is j + M/2 instead of M-1-j (cen_partner).
 //
 // Every 16-row output tile sums its k groups (4 rows each) from the group
 // farthest from the tile's diagonal to the nearest, so the large near-diagonal
@@ -639,6 +659,7 @@ struct SlabParams {
   const unsigned short* lo_z;
   const unsigned short* lo_y;
   int X, Y, Z;
+  int shift;       // centrosymmetric pairing (see cen_partner)
 };
 
 template <typename T, int MP, bool CEN>
@@ -654,6 +675,7 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
   const long base = (long)blockIdx.x * Y * Z;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   constexpr bool SPLIT = std::is_same<T, float>::value;
+  const bool shift = p.shift != 0;
 
   // ---- operands of both products, loaded with the plane ----
   const int jt = w % JT;  // every tile of this wave shares its row tile
@@ -695,7 +717,7 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
       const int f = tid + NT * k;
       const int y = f / H, z = f - (f / H) * H;
       g1[k] = base + (long)y * MP + z;
-      g2[k] = base + (long)y * MP + (MP - 1 - z);
+      g2[k] = base + (long)y * MP + cen_partner<MP>(z, shift);
       x1[k] = ch.in[g1[k]];
       x2[k] = ch.in[g2[k]];
     }
@@ -740,7 +762,8 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
 
   if constexpr (CEN) {
     // lds[y][j] holds U (j < H) and V (j >= H) along z; rebuild the z values
-    // and form the y pairs: lds[y'][z] = u, lds[H + y'][z] = v.
+    // and form the y pairs: lds[y'][z] = u, lds[H + y'][z] = v (z and the
+    // row y as in memory, the partners at cen_partner).
     constexpr int KB = (H * H + NT - 1) / NT;  // 2 x 2 blocks per thread
     T nv[KB][4];
 #pragma unroll
@@ -748,26 +771,26 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
       const int f = tid + NT * k;
       if (f >= H * H) continue;
       const int y = f / H, z = f - (f / H) * H;
-      const int yb = MP - 1 - y;
+      const int yb = cen_partner<MP>(y, shift);
       const T U1 = lds[y * P + z], V1 = lds[y * P + H + z];
       const T U2 = lds[yb * P + z], V2 = lds[yb * P + H + z];
-      const T t11 = U1 + V1, t12 = U1 - V1;  // (y, z), (y, MP-1-z)
-      const T t21 = U2 + V2, t22 = U2 - V2;  // (yb, z), (yb, MP-1-z)
+      const T t11 = U1 + V1, t12 = U1 - V1;  // (y, z), (y, zb)
+      const T t21 = U2 + V2, t22 = U2 - V2;  // (yb, z), (yb, zb)
       nv[k][0] = t11 + t21;  // u at (y, z)
-      nv[k][1] = t12 + t22;  // u at (y, MP-1-z)
+      nv[k][1] = t12 + t22;  // u at (y, zb)
       nv[k][2] = t11 - t21;  // v at (y, z)
-      nv[k][3] = t12 - t22;  // v at (y, MP-1-z)
+      nv[k][3] = t12 - t22;  // v at (y, zb)
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
       const int f = tid + NT * k;
       if (f >= H * H) continue;
-      const int y = f / H, z = f - (f / H) * H;
+      const int y = f / H, z = f - (f / H) * H, zb = cen_partner<MP>(z, shift);
       lds[y * P + z] = nv[k][0];
-      lds[y * P + MP - 1 - z] = nv[k][1];
+      lds[y * P + zb] = nv[k][1];
       lds[(H + y) * P + z] = nv[k][2];
-      lds[(H + y) * P + MP - 1 - z] = nv[k][3];
+      lds[(H + y) * P + zb] = nv[k][3];
     }
     __syncthreads();
   }
@@ -807,7 +830,8 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
       const int f = tid + NT * k;
       const int y = f / MP, z = f - (f / MP) * MP;  // y < H
       const T U = lds[y * P + z], V = lds[(H + y) * P + z];
-      const long g1 = base + (long)y * MP + z, g2 = base + (long)(MP - 1 - y) * MP + z;
+      const long g1 = base + (long)y * MP + z;
+      const long g2 = base + (long)cen_partner<MP>(y, shift) * MP + z;
       T o1 = U + V, o2 = U - V;
       if (ch.post) {
         o1 *= ch.post[g1];

@@ -90,6 +90,7 @@ struct BatchInst {
   const T* mat_c[3];                // half-size centrosymmetric operators (cen)
   const unsigned short* mat_c_lo[3];
   int pcls, cen;                    // phase length class (0: none), centrosymmetric
+  int shift;                        // 3m: half-period pairing (cen_partner)
   T *qf, *qb, *scr0, *scr1, *EA, *EB;
   T *scr0b, *scr1b;                 // second scratch field per chain (phases)
   double *wA, *wB, *ksi, *phiA, *phiB;
@@ -121,8 +122,11 @@ __global__ __launch_bounds__(256) void batch_pass_kernel(const BatchInst<T>* __r
   if (I.mat_c[AXIS]) {  // half-size product (the axis length is its class, 32 or 64)
     const T* mc = I.mat_c[AXIS];
     const unsigned short* lc = I.mat_c_lo[AXIS];
-    if (I.cls[AXIS] == 32) mfma_pass_tile<T, AXIS, 32, true>(ch, mc, lc, I.X, I.Y, I.Z, tidx, lds);
-    else mfma_pass_tile<T, AXIS, 64, true>(ch, mc, lc, I.X, I.Y, I.Z, tidx, lds);
+    const bool sh = I.shift != 0;
+    if (I.cls[AXIS] == 32)
+      mfma_pass_tile<T, AXIS, 32, true>(ch, mc, lc, I.X, I.Y, I.Z, tidx, lds, sh);
+    else
+      mfma_pass_tile<T, AXIS, 64, true>(ch, mc, lc, I.X, I.Y, I.Z, tidx, lds, sh);
     return;
   }
   const T* mat = I.mat[AXIS];
@@ -141,7 +145,7 @@ __global__ __launch_bounds__(256) void batch_pass_kernel(const BatchInst<T>* __r
 // with E_B while Ns-t-1 >= NsA; phase k writes scratch field k % 2.
 //
 // NT = 1024 threads hold every body but the dense class-64 one within 128
-// VGPRs (16 waves per CU); a batch with dense class-64 planes (3m, or P1
+// VGPRs (16 waves per CU); a batch with dense class-64 planes (reduced
 // lengths 49..63) runs the NT = 512 kernel, which holds all bodies.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void batch_phase_kernel(const BatchInst<T>* __restrict__ insts,
@@ -175,8 +179,11 @@ __global__ __launch_bounds__(NT) void batch_phase_kernel(const BatchInst<T>* __r
   if (I.cen) {
     const T *mr = I.mat_c[ra], *mc = I.mat_c[2];
     const unsigned short *lr = I.mat_c_lo[ra], *lc = I.mat_c_lo[2];
-    if (I.pcls == 32) plane_phase_body<T, 32, true, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
-    else plane_phase_body<T, 64, true, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
+    const bool sh = I.shift != 0;
+    if (I.pcls == 32)
+      plane_phase_body<T, 32, true, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds, sh);
+    else
+      plane_phase_body<T, 64, true, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds, sh);
     return;
   }
   const T *mr = I.mat[ra], *mc = I.mat[2];
@@ -399,7 +406,8 @@ int batch_setup(crys_batch* b) {
   // the one-launch phases need every instance's three axes in one class;
   // centrosymmetric products where all three lengths equal a class of 32/64
   // per axis: the half-size product applies where the axis length equals its
-  // class of 32 or 64 (PmmmDCT and P1 operators are centrosymmetric)
+  // class of 32 or 64 (PmmmDCT and P1 operators are centrosymmetric, 3m
+  // operators have the half-period pairing)
   std::vector<int> pcls(C, 0), cen(C, 0), cax(3 * C, 0);
   bool all = crys::phase_enabled();
   for (size_t i = 0; i < C; ++i) {
@@ -411,7 +419,7 @@ int batch_setup(crys_batch* b) {
     bool ce = pcls[i] != 0;
     for (int a = 0; a < 3; ++a) {
       const int ca = b->cls[3 * i + a];
-      cax[3 * i + a] = d.kind != CRYS_3M && crys::centro_enabled() && ca % 32 == 0 &&
+      cax[3 * i + a] = crys::centro_enabled() && ca % 32 == 0 &&
                        (int)(d.grid_log[a] / red) == ca;
       ce = ce && cax[3 * i + a];
     }
@@ -506,7 +514,7 @@ int batch_setup(crys_batch* b) {
         const size_t o = b->off_m[3 * i + a];
         std::vector<double> At(M * (size_t)M);
         for (size_t k = 0; k < At.size(); ++k) At[k] = (double)mats_d[o + k];
-        const std::vector<double> Cm = crys::centro_half(At, M);
+        const std::vector<double> Cm = crys::centro_half(At, M, d.kind == CRYS_3M);
         std::copy(Cm.begin(), Cm.end(), mc.begin() + b->off_mc[3 * i + a]);
         if (std::is_same<T, float>::value)
           for (size_t k = 0; k < Cm.size(); ++k)
@@ -545,6 +553,7 @@ int batch_setup(crys_batch* b) {
     }
     I.pcls = pcls[i];
     I.cen = cen[i];
+    I.shift = d.kind == CRYS_3M ? 1 : 0;
     I.NsA = (int)b->NsA[i];
     I.NsB = (int)b->NsB[i];
     I.Ns = I.NsA + I.NsB;

@@ -126,7 +126,7 @@ struct OpSet {
   double ds;
   void* dmat[3];     // device, element type of the solver's precision
   void* dmat_lo[3];  // FP32: bf16 bits of D - (float)D (split-precision product)
-  void* dmat_c[3];   // half-size centrosymmetric form (PmmmDCT/P1, M % 32 == 0) or null
+  void* dmat_c[3];   // half-size centrosymmetric form (M % 32 == 0; 3m: half-period pairing) or null
   void* dmat_c_lo[3];
   void* dmat_dL[3];  // dD/dL per axis, built on first use (cell stress)
   void* drad[3];     // radix-pass tables (radix_pass.hpp) when the radix passes are selected
@@ -220,8 +220,9 @@ int upload_matrix(const crys_solver* h, const std::vector<double>& At, void** ds
 // Builds and uploads the three per-axis operators (or their L derivatives).
 // For FP32 solvers `lo` (if not null) receives the bf16 bits of
 // D - (float)D, the low part of the split-precision product.  When `cen` is
-// not null it receives the half-size centrosymmetric forms (PmmmDCT and P1
-// axes of length 32, 64 or 128; null elsewhere).
+// not null it receives the half-size centrosymmetric forms (axes of length 32,
+// 64 or 128; null elsewhere), paired by the half-period translation for 3m
+// (centro_half).
 int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3] = nullptr,
                     void* cen[3] = nullptr, void* cen_lo[3] = nullptr) {
   for (int a = 0; a < 3; ++a) {
@@ -231,9 +232,8 @@ int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3
     else if (h->kind == CRYS_P1) At = crys::p1_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
     else At = crys::m3_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
     CHK(upload_matrix(h, At, &dst[a], lo ? &lo[a] : nullptr));
-    if (cen && h->kind != CRYS_3M && M % 32 == 0 && M <= 128 && M != 96 &&
-        crys::centro_enabled()) {
-      std::vector<double> C = crys::centro_half(At, M);
+    if (cen && M % 32 == 0 && M <= 128 && M != 96 && crys::centro_enabled()) {
+      std::vector<double> C = crys::centro_half(At, M, h->kind == CRYS_3M);
       CHK(upload_matrix(h, C, &cen[a], cen_lo ? &cen_lo[a] : nullptr));
     }
   }
@@ -285,6 +285,7 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
   p.X = X;
   p.Y = Y;
   p.Z = Z;
+  p.shift = h->kind == CRYS_3M;
   crys::RadixParams<T> rp{};
   rp.X = X;
   rp.Y = Y;
@@ -325,6 +326,7 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
     sp.X = X;
     sp.Y = Y;
     sp.Z = Z;
+    sp.shift = h->kind == CRYS_3M;
     const bool cen_zy = op.dmat_c[1] && op.dmat_c[2] && Y == Z;
     for (int c = 0; c < nch; ++c) sp.c[c] = {in[c], scr[c], E[c], nullptr};
     sp.mat_z = (const T*)(cen_zy ? op.dmat_c[2] : op.dmat[2]);
@@ -385,6 +387,7 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
   const bool cen = op.dmat_c[0] && op.dmat_c[1] && op.dmat_c[2] && X == mp && Y == mp && Z == mp;
   const bool yslab = (k % 2) == 1;
   crys::PhaseParams<T> p{};
+  p.shift = h->kind == CRYS_3M;
   const int ra = yslab ? 0 : 1;  // row axis
   p.mat_r = (const T*)(cen ? op.dmat_c[ra] : op.dmat[ra]);
   p.mat_c = (const T*)(cen ? op.dmat_c[2] : op.dmat[2]);

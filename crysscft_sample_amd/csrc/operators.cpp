@@ -87,12 +87,13 @@ std::vector<double> m3_axis_operator(int N, double L, double ds, bool dL) {
   return At;
 }
 
-std::vector<double> centro_half(const std::vector<double>& At, int M) {
+std::vector<double> centro_half(const std::vector<double>& At, int M, bool shift) {
   const int H = M / 2;
   std::vector<double> c((size_t)H * M);
   for (int k = 0; k < H; ++k)
     for (int j = 0; j < H; ++j) {
-      const double a = At[(size_t)k * M + j], b = At[(size_t)(M - 1 - k) * M + j];
+      const int kp = shift ? H + k : M - 1 - k;  // partner of k
+      const double a = At[(size_t)k * M + j], b = At[(size_t)kp * M + j];
       c[(size_t)k * M + j] = 0.5 * (a + b);
       c[(size_t)k * M + H + j] = 0.5 * (a - b);
     }

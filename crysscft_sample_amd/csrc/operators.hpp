@@ -36,7 +36,16 @@ std::vector<double> m3_axis_operator(int N_logical, double L, double ds, bool dL
 // true for PmmmDCT and P1): from At (M x M, At[k*M + j] = D[j][k]) returns
 // At_c (M/2 x M) with At_c[k*M + j] = Ps[j][k] for j < M/2 and
 // At_c[k*M + M/2 + j] = Pa[j][k], Ps/Pa = (D[j][k] +/- D[j][M-1-k]) / 2.
-std::vector<double> centro_half(const std::vector<double>& At, int M);
+//
+// With `shift` the pairing is the half-period translation k <-> k + M/2
+// instead (D[j+M/2][k+M/2] = D[j][k], indices mod M): the 3m operator with
+// zero translational parts (M even).  On the even sub-lattice the mirror
+// f(N-1-j) = f(j) composed with the translation by N/2 maps e[m] to
+// e[m + M/2], and D3 commutes with it.  In the basis that keeps k < M/2 and
+// reverses the upper half (position M-1-i holds index M/2 + i) D3 is
+// centrosymmetric; Ps/Pa = (D[j][k] +/- D[j][M/2+k]) / 2 are its blocks, and
+// the kernels read the partner of k at M/2 + k.
+std::vector<double> centro_half(const std::vector<double>& At, int M, bool shift = false);
 
 // Constant table of the radix pass (radix_pass.hpp) for one axis.  kind 0:
 // P1, transform length M = N (the logical length), K' = K(k folded) / N;

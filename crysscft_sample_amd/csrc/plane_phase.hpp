@@ -28,8 +28,9 @@
 // (16 rows x 4 contiguous columns, pitch PA): no access is transposed.
 // With CEN (centrosymmetric operators, all three axes of length MP, a
 // multiple of 32) both products use the half-size u/v split described at
-// SlabShape; the splits and recombinations are fused into the staging, the
-// E/history stage and the final store.
+// SlabShape, with the mirror or the half-period pairing (cen_partner); the
+// splits and recombinations are fused into the staging, the E/history stage
+// and the final store.
 #pragma once
 
 #include "axis_pass.hpp"
@@ -53,6 +54,7 @@ struct PhaseParams {
   const T* tab_r;  // radix tables of the row axis and z (length-128 radix phase), or null
   const T* tab_c;
   int rkind;       // RadixKind of the radix phase
+  int shift;       // centrosymmetric pairing (cen_partner in axis_pass.hpp)
   const unsigned short* lo_r;  // bf16 low parts (FP32), or null
   const unsigned short* lo_c;
   int MR, MC;           // row axis length, z length
@@ -188,7 +190,8 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
                                                  const T* mat_c,
                                                  const unsigned short* mat_r_lo,
                                                  const unsigned short* mat_c_lo, int MR, int MC,
-                                                 long base, long RS, T* lds) {
+                                                 long base, long RS, T* lds,
+                                                 bool shift = false) {
   using S = PhaseShape<T, MP, CEN, NT_>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int NT = S::NT, JT = S::JT, H = S::H, PB = S::PB, PA = S::PA, TPW = S::TPW;
@@ -206,7 +209,8 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
 
   // ---- every global load up front: the plane and the E values that the
   // E/history stage needs (same element per task), one memory latency ----
-  // CEN task k: rows r < H and M-1-r of column c; else task k: (r, c)
+  // CEN task k: rows r < H and its partner (cen_partner) of column c; else
+  // task k: (r, c)
   constexpr int TASKS = CEN ? H * MP : MP * MP;
   constexpr int KE = (TASKS + NT - 1) / NT;
   constexpr int NV = CEN ? 2 : 1;  // elements per task
@@ -219,7 +223,7 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
     const int r = f / MP, c = f - r * MP;
     ok[k] = f < TASKS && (CEN || (r < MR && c < MC));
     g[k][0] = base + r * RS + c;
-    if constexpr (CEN) g[k][1] = base + (MP - 1 - r) * RS + c;
+    if constexpr (CEN) g[k][1] = base + cen_partner<MP>(r, shift) * RS + c;
   }
   // the plane first, then one E (the finished step's, or the next step's in
   // a start phase), so that staging waits only for the plane; the other E of
@@ -240,7 +244,7 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
     const int f = tid + NT * k;
     const int r = f / MP, c = f - r * MP;
     if constexpr (CEN) {
-      // rows r < H: u = x_r + x_{M-1-r}; rows H + r: v = x_r - x_{M-1-r}
+      // rows r < H: u = x_r + x_{p(r)}; rows H + r: v = x_r - x_{p(r)}
       if (f >= TASKS) continue;
       const T a = fin ? x[k][0] : x[k][0] * ef[k][0];
       const T b = fin ? x[k][1] : x[k][1] * ef[k][1];
@@ -318,19 +322,20 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
       }
     }
     __syncthreads();
-    // rows U (j < H) / V (H + j) -> values w at rows j, M-1-j; then the z
-    // pairs of each row: A operand [row][u (c < H) | v (H + c)] at pitch PA
+    // rows U (j < H) / V (H + j) -> values w at rows j, M-1-j of the paired
+    // basis; then the z pairs of each row (column c and its partner cp in
+    // memory order): A operand [row][u (c < H) | v (H + c)] at pitch PA
     constexpr int KB = (H * H + NT - 1) / NT;
     T nv[KB][4];
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
       const int f = tid + NT * k;
       if (f >= H * H) continue;
-      const int j = f / H, c = f - (f / H) * H;
+      const int j = f / H, c = f - (f / H) * H, cp = cen_partner<MP>(c, shift);
       const T P1 = lds[j * PB + c], P2 = lds[(H + j) * PB + c];
-      const T P3 = lds[j * PB + MP - 1 - c], P4 = lds[(H + j) * PB + MP - 1 - c];
+      const T P3 = lds[j * PB + cp], P4 = lds[(H + j) * PB + cp];
       const T w11 = P1 + P2, w21 = P1 - P2;  // (j, c), (M-1-j, c)
-      const T w12 = P3 + P4, w22 = P3 - P4;  // (j, M-1-c), (M-1-j, M-1-c)
+      const T w12 = P3 + P4, w22 = P3 - P4;  // (j, cp), (M-1-j, cp)
       nv[k][0] = w11 + w12;  // u (j, c)
       nv[k][1] = w11 - w12;  // v (j, c)
       nv[k][2] = w21 + w22;  // u (M-1-j, c)
@@ -385,8 +390,9 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
       if (f >= TASKS) continue;
       const int r = f / H, j = f - (f / H) * H;
       const T U = lds[r * PB + j], V = lds[r * PB + H + j];
-      __builtin_nontemporal_store(U + V, &ch.out[base + r * RS + j]);
-      __builtin_nontemporal_store(U - V, &ch.out[base + r * RS + MP - 1 - j]);
+      const long g0 = base + cen_index<MP>(r, shift) * RS;
+      __builtin_nontemporal_store(U + V, &ch.out[g0 + j]);
+      __builtin_nontemporal_store(U - V, &ch.out[g0 + cen_partner<MP>(j, shift)]);
     }
   } else {
 #pragma unroll
@@ -403,7 +409,8 @@ template <typename T, int MP, bool CEN, int NT_ = 512>
 __global__ __launch_bounds__(NT_) void plane_phase_kernel(PhaseParams<T> p) {
   __shared__ T lds[MP * PhaseShape<T, MP, CEN, NT_>::PB];
   plane_phase_body<T, MP, CEN, NT_>(p.c[blockIdx.y], p.mat_r, p.mat_c, p.lo_r, p.lo_c, p.MR, p.MC,
-                                    (long)blockIdx.x * p.plane_stride, p.row_stride, lds);
+                                    (long)blockIdx.x * p.plane_stride, p.row_stride, lds,
+                                    p.shift != 0);
 }
 
 // One 16 x 16 output tile with the plane as the B operand: out[j][c] =
@@ -564,7 +571,8 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a_khalf(
 // whole and apply the finishing row product whole (the next step's row
 // product needs every row of its result); each then writes the history rows,
 // applies the next row product and the z product, and stores only for its
-// half h of the plane: the rows j and M-1-j with j in [16 h, 16 h + 16), i.e.
+// half h of the plane: the rows j and M-1-j (positions of the paired basis,
+// see cen_index) with j in [16 h, 16 h + 16), i.e.
 // the U/V row tiles h and 2 + h and the output row tiles h and 3 - h.  Per
 // CU this is 2 product units instead of 3.  The partners of a plane have
 // workgroup ids 8 apart, which places them on one XCD (shared L2).
@@ -606,6 +614,7 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
   const bool lo_r = SPLIT && p.lo_r != nullptr, lo_c = SPLIT && p.lo_c != nullptr;
+  const bool shift = p.shift != 0;
 
   // operators: finishing row product (row tile w % 4, all k); next row
   // product (row tile h or 2 + h, k half kh); z product (output column tile
@@ -640,7 +649,7 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     const int f = tid + NT * k;
     const int r = f / MP, c = f - r * MP;
     g1[k] = base + r * RS + c;
-    g2[k] = base + (MP - 1 - r) * RS + c;
+    g2[k] = base + cen_partner<MP>(r, shift) * RS + c;
   }
   // the plane first, then one E (the finished step's, or the next step's in
   // a start phase): unconditional loads in a fixed order, so that staging
@@ -734,15 +743,17 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
       lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
-  // ---- rows j, M-1-j of this half (j = 16 h + f / 32) -> z pairs at PA ----
+  // ---- rows j, M-1-j of this half (j = 16 h + f / 32) -> z pairs at PA
+  // (column c and its partner cp in memory order) ----
   const bool mid = tid < 16 * H;
   if (mid) {
     const int j = 16 * h + tid / H, c = tid - (tid / H) * H, jo = j ^ 16;
+    const int cp = cen_partner<MP>(c, shift);
     T nv[4];
     const T P1 = buf1[j * PB + c] + buf1[jo * PB + c];
     const T P2 = buf1[(H + j) * PB + c] + buf1[(H + jo) * PB + c];
-    const T P3 = buf1[j * PB + MP - 1 - c] + buf1[jo * PB + MP - 1 - c];
-    const T P4 = buf1[(H + j) * PB + MP - 1 - c] + buf1[(H + jo) * PB + MP - 1 - c];
+    const T P3 = buf1[j * PB + cp] + buf1[jo * PB + cp];
+    const T P4 = buf1[(H + j) * PB + cp] + buf1[(H + jo) * PB + cp];
     const T w11 = P1 + P2, w21 = P1 - P2;
     const T w12 = P3 + P4, w22 = P3 - P4;
     nv[0] = w11 + w12;
@@ -772,8 +783,9 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     const int ro = row ^ 16;
     const T U = buf1[row * PB + j] + buf1[ro * PB + j];
     const T V = buf1[row * PB + H + j] + buf1[ro * PB + H + j];
-    __builtin_nontemporal_store(U + V, &ch.out[base + row * RS + j]);
-    __builtin_nontemporal_store(U - V, &ch.out[base + row * RS + MP - 1 - j]);
+    const long g0 = base + cen_index<MP>(row, shift) * RS;
+    __builtin_nontemporal_store(U + V, &ch.out[g0 + j]);
+    __builtin_nontemporal_store(U - V, &ch.out[g0 + cen_partner<MP>(j, shift)]);
   }
 }
 

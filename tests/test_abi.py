@@ -96,6 +96,39 @@ def test_axis_operators_reproduce_the_oracle(oracle, name, kind, N, cell, ds):
     assert rel_err(tensor_apply(f, Ds), s.apply(f)) < 1e-14
 
 
+@pytest.mark.parametrize("name,kind,pair", [("PmmmDCT", _lib.PMMM_DCT, "mirror"),
+                                            ("P1", _lib.P1, "mirror"), ("3m", _lib.M3, "shift")])
+@pytest.mark.parametrize("N", [64, 128])
+def test_half_size_products_pair_as_the_kernels_do(name, kind, pair, N):
+    """The half-size products split a line into u_i = x_i + x_p(i), v_i = x_i -
+    x_p(i) (i < M/2) with p(i) = M-1-i (PmmmDCT, P1) or i + M/2 (3m), apply
+    Ps = (D[j][k] + D[j][p(k)])/2 and Pa = (D[j][k] - D[j][p(k)])/2, and write
+    U+V at j and U-V at p(j).  That equals D exactly only when D commutes with
+    the pairing: each solver's operator has its own symmetry and not the
+    other one (D itself is pinned to the oracle above)."""
+    ds, L = 0.01, 4.2
+    At = C.axis_operator(kind, N, L, ds)
+    D = At.T
+    M = D.shape[0]
+    H = M // 2
+    mirror = np.arange(M)[::-1]
+    shift = (np.arange(M) + H) % M
+    sym = {"mirror": D[np.ix_(mirror, mirror)], "shift": D[np.ix_(shift, shift)]}
+    other = "shift" if pair == "mirror" else "mirror"
+    assert np.abs(sym[pair] - D).max() <= 1e-15 * np.abs(D).max()
+    if name != "P1":  # a circulant commutes with both
+        assert np.abs(sym[other] - D).max() > 1e-3 * np.abs(D).max()
+    p = (mirror if pair == "mirror" else shift)[:H]
+    k = np.arange(H)
+    Ps = 0.5 * (D[np.ix_(k, k)] + D[np.ix_(k, p)])
+    Pa = 0.5 * (D[np.ix_(k, k)] - D[np.ix_(k, p)])
+    x = np.random.default_rng(N).standard_normal(M)
+    U, V = Ps @ (x[:H] + x[p]), Pa @ (x[:H] - x[p])
+    got = np.empty(M)
+    got[:H], got[p] = U + V, U - V
+    assert rel_err(got, D @ x) < 1e-14
+
+
 def test_fields_are_mirror_symmetric_and_roundtrip():
     wA, wB = F.bcc_fields((16, 16, 16), 4.0, 0.25, 20.0)
     for ax in range(3):

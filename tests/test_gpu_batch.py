@@ -34,6 +34,7 @@ MIXED = [
     Instance("P1", (16, 24, 32), (4.0, 4.1, 4.3), 0.28, 22.0, 1 / 128),
     Instance("PmmmDCT", (64, 64, 64), (4.2,) * 3, 0.3, 25.0, 0.005),
     Instance("PmmmDCT", (32, 48, 16), (3.7, 4.0, 4.4), 0.22, 18.0, 1 / 120, orthorhombic=True),
+    Instance("3m", (64, 64, 64), (4.1,) * 3, 0.27, 23.0, 1 / 90),  # half-period pairing, 32
 ]
 
 
@@ -99,10 +100,12 @@ PHASE_BATCH = [
     Instance("PmmmDCT", (32, 32, 32), (4.0,) * 3, 0.25, 20.0, 0.01),        # class 16
     Instance("PmmmDCT", (64, 64, 64), (4.2,) * 3, 0.3, 25.0, 1 / 150),       # 32, centrosymmetric
     Instance("P1", (32, 32, 32), (4.1,) * 3, 0.28, 22.0, 1 / 120),           # 32, centrosymmetric
-    Instance("3m", (32, 32, 32), (3.9,) * 3, 0.2, 30.0, 0.01),               # 16 (3m: dense)
+    Instance("3m", (32, 32, 32), (3.9,) * 3, 0.2, 30.0, 0.01),               # 16
     Instance("PmmmDCT", (96, 96, 96), (4.4,) * 3, 0.31, 24.0, 0.02),         # 48
     Instance("PmmmDCT", (128, 128, 128), (4.2,) * 3, 0.3, 25.0, 0.01),       # 64, centrosymmetric
     Instance("P1", (40, 40, 40), (4.0,) * 3, 0.33, 18.0, 1 / 80),            # 48, padded
+    Instance("3m", (64, 64, 64), (4.1,) * 3, 0.27, 23.0, 1 / 90),            # 32, half-period pairing
+    Instance("3m", (128, 128, 128), (4.3,) * 3, 0.32, 26.0, 0.02),           # 64, half-period pairing
 ]
 
 

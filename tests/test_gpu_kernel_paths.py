@@ -4,7 +4,8 @@ The iteration picks its contour-step kernels by grid: the one-launch plane
 phases (split over two workgroups per plane at class 64, or one; the radix
 plane phase when every axis has length 128), the fused z+y slab (matrix-core,
 or radix at length 128) plus the x pass, or three axis passes; the
-centrosymmetric products where they apply.  The environment switches
+centrosymmetric products where they apply (mirror pairing for PmmmDCT and
+P1, half-period pairing for 3m).  The environment switches
 CRYS_PHASE, CRYS_PHASE_HALF, CRYS_SLAB, CRYS_CENTRO and CRYS_AXIS_PASS force
 the other paths; each is read once per process, so each case runs the GPU
 side in a child process.  FP64 traces must stay within 1e-10 of the oracle on
@@ -40,9 +41,11 @@ print(json.dumps(run.run(nstep).tolist()))
 """
 
 # (solver, logical grid, cell, ds): length classes 64 and 48, and length 128
-# on every axis (radix phase / radix slab / radix passes / matrix-core passes)
+# on every axis (radix phase / radix slab / radix passes / matrix-core passes);
+# 3m at classes 64 and 32 (half-size products with the half-period pairing)
 CASES = [("PmmmDCT", (128, 128, 128), (4.2,) * 3, 0.01), ("P1", (48, 48, 48), (4.2,) * 3, 0.01),
-         ("PmmmDCT", (256, 256, 256), (5.0,) * 3, 0.05)]
+         ("PmmmDCT", (256, 256, 256), (5.0,) * 3, 0.05), ("3m", (128, 128, 128), (4.2,) * 3, 0.01),
+         ("3m", (64, 64, 64), (4.0,) * 3, 0.01)]
 SWITCHES = [{}, {"CRYS_PHASE_HALF": "0"}, {"CRYS_PHASE": "0"},
             {"CRYS_PHASE": "0", "CRYS_SLAB": "0"}, {"CRYS_CENTRO": "0"},
             {"CRYS_AXIS_PASS": "mfma"}]
