@@ -653,12 +653,15 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   }
   // the plane first, then one E (the finished step's, or the next step's in
   // a start phase): unconditional loads in a fixed order, so that staging
-  // waits only for the plane; the other E of a block junction is read later
+  // waits only for the plane; the other E of a block junction is read later.
+  // The plane is read once, non-temporally, so that it does not displace the
+  // E values (read again by every step) from this XCD's L2: measured 0.7% to
+  // 2.2% faster per FP64 step at 64^3 reduced (two machines), FP32 within 0.5%.
   const T* e_a = fin ? ch.e_fin : ch.e_next;
 #pragma unroll
   for (int k = 0; k < KE; ++k) {
-    x1[k] = ch.in[g1[k]];
-    x2[k] = ch.in[g2[k]];
+    x1[k] = __builtin_nontemporal_load(&ch.in[g1[k]]);
+    x2[k] = __builtin_nontemporal_load(&ch.in[g2[k]]);
   }
 #pragma unroll
   for (int k = 0; k < KE; ++k) {
@@ -776,7 +779,9 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
       lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
-  // ---- store this half's rows (non-temporal, as in plane_phase_kernel) ----
+  // ---- store this half's rows: non-temporal for FP64 as in
+  // plane_phase_kernel; ordinary FP32 stores measured about 1% faster per
+  // step, ordinary FP64 stores 2% slower ----
   {
     const int rr = tid / H, j = tid - (tid / H) * H;
     const int row = rr < 16 ? 16 * h + rr : 48 - 16 * h + (rr - 16);
@@ -784,8 +789,14 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     const T U = buf1[row * PB + j] + buf1[ro * PB + j];
     const T V = buf1[row * PB + H + j] + buf1[ro * PB + H + j];
     const long g0 = base + cen_index<MP>(row, shift) * RS;
-    __builtin_nontemporal_store(U + V, &ch.out[g0 + j]);
-    __builtin_nontemporal_store(U - V, &ch.out[g0 + cen_partner<MP>(j, shift)]);
+    const long g3 = g0 + cen_partner<MP>(j, shift);
+    if constexpr (SPLIT) {
+      ch.out[g0 + j] = U + V;
+      ch.out[g3] = U - V;
+    } else {
+      __builtin_nontemporal_store(U + V, &ch.out[g0 + j]);
+      __builtin_nontemporal_store(U - V, &ch.out[g3]);
+    }
   }
 }
 

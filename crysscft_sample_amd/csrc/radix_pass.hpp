@@ -776,7 +776,12 @@ __global__ __launch_bounds__(512) void radix_phase_kernel(PhaseParams<T> p) {
   {
     T v[KT];
 #pragma unroll
-    for (int k = 0; k < KT; ++k) v[k] = in_b[gidx(tid + NT * k)];
+    for (int k = 0; k < KT; ++k) {
+      // FP32: the plane is read once, non-temporally (measured 3% faster per
+      // step at 128^3; FP64 unchanged within noise)
+      if constexpr (sizeof(T) == 4) v[k] = __builtin_nontemporal_load(&in_b[gidx(tid + NT * k)]);
+      else v[k] = in_b[gidx(tid + NT * k)];
+    }
     if (!fin) load_e(enext_b);
     else if (E_EARLY) load_e(efin_b);
 #pragma unroll
