@@ -49,10 +49,10 @@ STEP_KERNELS = {
         "crys::axis_pass_mfma_kernel<double,0,64>"],
 }
 # HBM bytes per dispatch of both chains, from the rocprofv3 FETCH_SIZE and
-# WRITE_SIZE passes of this workload (FETCH_SIZE scaled by 1.983 from a copy
+# WRITE_SIZE passes of this workload (FETCH_SIZE scaled by 1.975 from a copy
 # of known size), profiles/hbm_traffic_plane_phase_pmmm128_fp64.md.
 MEASURED_TRAFFIC = {
-    "crys::plane_phase_half_kernel<double>": 16.21e6,
+    "crys::plane_phase_half_kernel<double>": 16.18e6,
     "crys::plane_phase_kernel<double,64,true,1024>": 16.17e6,
     "crys::slab_zy_kernel<double,64,true>": 12.00e6,
     "crys::axis_pass_mfma_kernel<double,0,64,true>": 11.86e6,
