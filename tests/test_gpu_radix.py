@@ -112,10 +112,11 @@ import numpy as np
 sys.path.insert(0, {root!r})
 import crysscft_sample_amd as C
 from crysscft_sample_amd import fields as F
+name = sys.argv[1]
 N, cell, fA, xN, ds = (256,) * 3, (5.0,) * 3, 0.3, 20.0, 0.05
 wA, wB = F.bcc_fields(N, cell, fA, xN)
-a, b = F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT")
-s = C.SolverPmmmDCT(N)
+a, b = F.extract(wA, name), F.extract(wB, name)
+s = C.SOLVERS[name](N)
 run = C.SCFT(s, fA, ds, xN, cell, 0.1)
 g = run.evaluate(a, b, stress=True)
 print(json.dumps({{"H": g["H"], "dL": g["dH_dL"].tolist(),
@@ -124,13 +125,17 @@ print(json.dumps({{"H": g["H"], "dL": g["dH_dL"].tolist(),
 """
 
 
-def test_radix_phase_evaluate_matches_matrix_cores():
+@pytest.mark.parametrize("name", ["PmmmDCT", "3m"])
+def test_radix_phase_evaluate_matches_matrix_cores(name):
     """crys_scft_evaluate (H, dH/dw, stress) at 256^3 through the radix plane
-    phase equals the matrix-core passes (CRYS_AXIS_PASS=mfma) to 1e-10."""
+    phase equals the matrix-core passes (CRYS_AXIS_PASS=mfma: half-size
+    products at length 128, mirror-paired for PmmmDCT, half-period-paired for
+    3m) to 1e-10."""
     out = {}
     for label, env_add in (("radix", {}), ("mfma", {"CRYS_AXIS_PASS": "mfma"})):
         env = dict(os.environ, **env_add)
-        r = subprocess.run([sys.executable, "-c", EVAL_CHILD.format(root=ROOT)], env=env,
+        print("evaluate 256^3", name, label, file=sys.stderr, flush=True)
+        r = subprocess.run([sys.executable, "-c", EVAL_CHILD.format(root=ROOT), name], env=env,
                            capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr[-2000:]
         out[label] = json.loads(r.stdout.strip().splitlines()[-1])
