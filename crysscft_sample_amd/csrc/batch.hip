@@ -255,10 +255,15 @@ __global__ __launch_bounds__(kRedThreads) void batch_phi_kernel(const BatchInst<
       double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
       long s = s0 + 1;
       for (; s + 4 <= s1; s += 4) {
-        const T a0 = I.qf[s * n + i], a1 = I.qf[(s + 1) * n + i];
-        const T a2 = I.qf[(s + 2) * n + i], a3 = I.qf[(s + 3) * n + i];
-        const T b0 = I.qb[s * n + i], b1 = I.qb[(s + 1) * n + i];
-        const T b2 = I.qb[(s + 2) * n + i], b3 = I.qb[(s + 3) * n + i];
+        // each history entry is read once: non-temporal loads
+        const T a0 = __builtin_nontemporal_load(&I.qf[s * n + i]);
+        const T a1 = __builtin_nontemporal_load(&I.qf[(s + 1) * n + i]);
+        const T a2 = __builtin_nontemporal_load(&I.qf[(s + 2) * n + i]);
+        const T a3 = __builtin_nontemporal_load(&I.qf[(s + 3) * n + i]);
+        const T b0 = __builtin_nontemporal_load(&I.qb[s * n + i]);
+        const T b1 = __builtin_nontemporal_load(&I.qb[(s + 1) * n + i]);
+        const T b2 = __builtin_nontemporal_load(&I.qb[(s + 2) * n + i]);
+        const T b3 = __builtin_nontemporal_load(&I.qb[(s + 3) * n + i]);
         p0 += (double)a0 * (double)b0;
         p1 += (double)a1 * (double)b1;
         p2 += (double)a2 * (double)b2;

@@ -153,7 +153,8 @@ __global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
     // weigh 1/2); R slots per round with independent partial sums, so that
     // 2 R loads are in flight per point (the grid holds 8 waves per CU):
     // R = 8 for FP32 (measured 91 -> 77 us at 128^3), 4 for FP64 (8 was
-    // slower)
+    // slower).  Each history entry is read once here, non-temporally: the
+    // FP64 iteration at 128^3 measured 1.7% faster than with ordinary loads.
     auto contour_sum = [&](size_t s0, size_t s1) -> double {
       constexpr int R = sizeof(T) == 4 ? 8 : 4;
       double p[R];
@@ -164,8 +165,8 @@ __global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
         T a[R], b[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          a[r] = qf[(s + r) * n + i];
-          b[r] = qb[(s + r) * n + i];
+          a[r] = __builtin_nontemporal_load(&qf[(s + r) * n + i]);
+          b[r] = __builtin_nontemporal_load(&qb[(s + r) * n + i]);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) p[r] += (double)a[r] * (double)b[r];
