@@ -381,8 +381,10 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
   }
   __syncthreads();
 
-  // ---- store the scratch plane (non-temporal: the next phase reads it from
-  // other XCDs, so it gains nothing from this XCD's L2) ----
+  // ---- store the scratch plane: non-temporal for FP64 (the next phase reads
+  // it from other XCDs, so it gains nothing from this XCD's L2); ordinary
+  // stores for FP32, measured 6% faster per step at 32^3 reduced and 1.5%
+  // at class 48 ----
   if constexpr (CEN) {
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
@@ -391,16 +393,24 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
       const int r = f / H, j = f - (f / H) * H;
       const T U = lds[r * PB + j], V = lds[r * PB + H + j];
       const long g0 = base + cen_index<MP>(r, shift) * RS;
-      __builtin_nontemporal_store(U + V, &ch.out[g0 + j]);
-      __builtin_nontemporal_store(U - V, &ch.out[g0 + cen_partner<MP>(j, shift)]);
+      const long g3 = g0 + cen_partner<MP>(j, shift);
+      if constexpr (SPLIT) {
+        ch.out[g0 + j] = U + V;
+        ch.out[g3] = U - V;
+      } else {
+        __builtin_nontemporal_store(U + V, &ch.out[g0 + j]);
+        __builtin_nontemporal_store(U - V, &ch.out[g3]);
+      }
     }
   } else {
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
       const int f = tid + NT * k;
       const int r = f / MP, c = f - r * MP;
-      if (f < TASKS && r < MR && c < MC)
-        __builtin_nontemporal_store(lds[r * PB + c], &ch.out[base + r * RS + c]);
+      if (f < TASKS && r < MR && c < MC) {
+        if constexpr (SPLIT) ch.out[base + r * RS + c] = lds[r * PB + c];
+        else __builtin_nontemporal_store(lds[r * PB + c], &ch.out[base + r * RS + c]);
+      }
     }
   }
 }
