@@ -295,6 +295,22 @@ __device__ __forceinline__ void radix_fused_bsa(T* buf, const T* tw, const T* kp
   }
 }
 
+// Global field traffic of the radix passes and the radix slab: each value is
+// read and written once per launch.  FP32 uses non-temporal loads and stores,
+// so that E and the tables stay in L2 (measured 7-11% faster per FP32 step at
+// 128^3); FP64 keeps ordinary accesses (non-temporal ones measured 1-2.5%
+// slower on three of four workloads).
+template <typename T>
+__device__ __forceinline__ T radix_line_load(const T* p) {
+  if constexpr (sizeof(T) == 4) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <typename T>
+__device__ __forceinline__ void radix_line_store(T v, T* p) {
+  if constexpr (sizeof(T) == 4) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // Waves per SIMD the compiler must allow: 4 (four 256-thread workgroups per
 // CU) where that fits in 128 VGPRs without spilling; the FP64 length-128
 // kernel needs about 150 and measured slower when held to 128 (spills).
@@ -363,7 +379,7 @@ __attribute__((amdgpu_waves_per_eu(radix_waves_per_eu<T, M>()))) void radix_pass
     }
     T v[KT];
 #pragma unroll
-    for (int k = 0; k < KT; ++k) v[k] = ok[k] ? ch.in[gi[k]] : T(0);
+    for (int k = 0; k < KT; ++k) v[k] = ok[k] ? radix_line_load(&ch.in[gi[k]]) : T(0);
     if (ch.pre) {
       T e[KT];
 #pragma unroll
@@ -463,7 +479,7 @@ __attribute__((amdgpu_waves_per_eu(radix_waves_per_eu<T, M>()))) void radix_pass
   }
 #pragma unroll
   for (int k = 0; k < KT; ++k)
-    if (ok[k]) ch.out[gi[k]] = v[k];
+    if (ok[k]) radix_line_store(v[k], &ch.out[gi[k]]);
 }
 
 // ---------------------------------------------------------------------------
@@ -670,7 +686,7 @@ __global__ __launch_bounds__(RadixSlabShape<T>::NT) void radix_slab_kernel(Radix
   for (int b = 0; b < NBAT; ++b) {
     T v[KB];
 #pragma unroll
-    for (int k = 0; k < KB; ++k) v[k] = ch.in[base + tid + NT * (b * KB + k)];
+    for (int k = 0; k < KB; ++k) v[k] = radix_line_load(&ch.in[base + tid + NT * (b * KB + k)]);
     if (ch.pre) {
       T e[KB];
 #pragma unroll
@@ -699,7 +715,7 @@ __global__ __launch_bounds__(RadixSlabShape<T>::NT) void radix_slab_kernel(Radix
       for (int k = 0; k < KB; ++k) v[k] *= e[k];
     }
 #pragma unroll
-    for (int k = 0; k < KB; ++k) ch.out[base + tid + NT * (b * KB + k)] = v[k];
+    for (int k = 0; k < KB; ++k) radix_line_store(v[k], &ch.out[base + tid + NT * (b * KB + k)]);
   }
 }
 
