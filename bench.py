@@ -44,9 +44,11 @@ MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 STEP_KERNELS = {
     1: ["crys::plane_phase_half_kernel<double>" if os.environ.get("CRYS_PHASE_HALF") != "0"
         else "crys::plane_phase_kernel<double,64,true,1024>"],
-    2: ["crys::slab_zy_kernel<double,64,true>", "crys::axis_pass_mfma_kernel<double,0,64,true>"],
-    3: ["crys::axis_pass_mfma_kernel<double,2,64>", "crys::axis_pass_mfma_kernel<double,1,64>",
-        "crys::axis_pass_mfma_kernel<double,0,64>"],
+    2: ["crys::slab_zy_kernel<double,64,true,true>",
+        "crys::axis_pass_mfma_kernel<double,0,64,true,true>"],
+    3: ["crys::axis_pass_mfma_kernel<double,2,64,true,true>",
+        "crys::axis_pass_mfma_kernel<double,1,64,true,true>",
+        "crys::axis_pass_mfma_kernel<double,0,64,true,true>"],
 }
 # HBM bytes per dispatch of both chains, from the rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes of this workload (FETCH_SIZE scaled by 1.975 from a copy
@@ -54,8 +56,9 @@ STEP_KERNELS = {
 MEASURED_TRAFFIC = {
     "crys::plane_phase_half_kernel<double>": 16.18e6,
     "crys::plane_phase_kernel<double,64,true,1024>": 16.17e6,
-    "crys::slab_zy_kernel<double,64,true>": 12.00e6,
-    "crys::axis_pass_mfma_kernel<double,0,64,true>": 11.86e6,
+    # measured before these two kernels read and wrote non-temporally
+    "crys::slab_zy_kernel<double,64,true,true>": 12.00e6,
+    "crys::axis_pass_mfma_kernel<double,0,64,true,true>": 11.86e6,
 }
 
 

@@ -42,6 +42,7 @@ struct PassParams {
   int X, Y, Z;
   int cen;                      // 1: mat is the half-size centrosymmetric operator
   int shift;                    // with cen: half-period pairing (see cen_partner)
+  int stream;                   // 1: non-temporal field traffic (field_load)
 };
 
 // Pairing of the half-size (centrosymmetric) products.  A line is split into
@@ -58,6 +59,21 @@ __device__ __forceinline__ int cen_partner(int i, bool shift) {
 template <int MP>
 __device__ __forceinline__ int cen_index(int i, bool shift) {
   return (shift && i >= MP / 2) ? (3 * MP / 2 - 1) - i : i;
+}
+
+// Field loads and stores of a pass: non-temporal when NT (the launch streams
+// fields that the next launch reads from other XCDs, as in a propagation),
+// ordinary otherwise (a lone diffusion() re-reads one field that stays in
+// L2).
+template <bool NT, typename T>
+__device__ __forceinline__ T field_load(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void field_store(T v, T* p) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
 }
 
 // Register micro-tile width along the lines for a padded length MP.
@@ -296,7 +312,7 @@ struct MfmaShape {
 // half-size operator At_c (M/2 x M) described at SlabShape, paired as
 // cen_partner(., shift) says.  The k groups of every output tile are summed
 // far-first (see SlabShape).
-template <typename T, int AXIS, int MP, bool CEN = false>
+template <typename T, int AXIS, int MP, bool CEN = false, bool NTM = true>
 __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* __restrict__ mat,
                                                const unsigned short* __restrict__ mat_lo, int X,
                                                int Y, int Z, int tidx, T* tile,
@@ -376,14 +392,15 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
     if constexpr (SPLIT) alo[n] = (okm && mat_lo) ? (short)mat_lo[i * ldm + jrow] : (short)0;
   }
 
-  // ---- stage (issue every global load first).  The lines are read and
-  // written once per pass, non-temporally, so that the E values and the
-  // operators stay in L2: measured 7% faster FP64 (3% FP32) on a batch of
-  // 256 instances, whose contour steps are these passes ----
+  // ---- stage (issue every global load first).  With NTM the lines are read
+  // and written non-temporally, so that the E values and the operators stay
+  // in L2: measured 7% faster FP64 (3% FP32) on a batch of 256 instances,
+  // whose contour steps are these passes; a lone diffusion() (NTM off) measured
+  // 4% slower with it ----
   if constexpr (!CEN) {
     T v[KT];
 #pragma unroll
-    for (int k = 0; k < KT; ++k) v[k] = ok[k] ? __builtin_nontemporal_load(&ch.in[gidx(si[k], sl[k])]) : T(0);
+    for (int k = 0; k < KT; ++k) v[k] = ok[k] ? field_load<NTM>(&ch.in[gidx(si[k], sl[k])]) : T(0);
     if (ch.pre) {
       T e[KT];
 #pragma unroll
@@ -398,8 +415,8 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
     T x1[KT], x2[KT];
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
-      x1[k] = ok[k] ? __builtin_nontemporal_load(&ch.in[gidx(si[k], sl[k])]) : T(0);
-      x2[k] = ok[k] ? __builtin_nontemporal_load(&ch.in[gidx(cen_partner<MP>(si[k], shift), sl[k])]) : T(0);
+      x1[k] = ok[k] ? field_load<NTM>(&ch.in[gidx(si[k], sl[k])]) : T(0);
+      x2[k] = ok[k] ? field_load<NTM>(&ch.in[gidx(cen_partner<MP>(si[k], shift), sl[k])]) : T(0);
     }
     if (ch.pre) {
       T e1[KT], e2[KT];
@@ -480,7 +497,7 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
     }
 #pragma unroll
     for (int k = 0; k < KT; ++k)
-      if (ok[k]) __builtin_nontemporal_store(v[k], &ch.out[gidx(si[k], sl[k])]);
+      if (ok[k]) field_store<NTM>(v[k], &ch.out[gidx(si[k], sl[k])]);
   } else {
     T o1[KT], o2[KT];
 #pragma unroll
@@ -506,17 +523,26 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
 #pragma unroll
     for (int k = 0; k < KT; ++k)
       if (ok[k]) {
-        __builtin_nontemporal_store(o1[k], &ch.out[gidx(si[k], sl[k])]);
-        __builtin_nontemporal_store(o2[k], &ch.out[gidx(cen_partner<MP>(si[k], shift), sl[k])]);
+        field_store<NTM>(o1[k], &ch.out[gidx(si[k], sl[k])]);
+        field_store<NTM>(o2[k], &ch.out[gidx(cen_partner<MP>(si[k], shift), sl[k])]);
       }
   }
 }
 
-template <typename T, int AXIS, int MP, bool CEN = false>
+template <typename T, int AXIS, int MP, bool CEN, bool NTM>
 __global__ __launch_bounds__(MfmaShape<MP>::NT) void axis_pass_mfma_kernel(PassParams<T> p) {
   __shared__ T tile[MP * MfmaShape<MP>::LDT];
-  mfma_pass_tile<T, AXIS, MP, CEN>(p.c[blockIdx.y], p.mat, p.mat_lo, p.X, p.Y, p.Z, blockIdx.x,
-                                   tile, p.shift != 0);
+  mfma_pass_tile<T, AXIS, MP, CEN, NTM>(p.c[blockIdx.y], p.mat, p.mat_lo, p.X, p.Y, p.Z,
+                                        blockIdx.x, tile, p.shift != 0);
+}
+
+// Whether a matrix-core pass reads and writes its lines non-temporally: in a
+// propagation (stream), and always for FP32, whose lone diffusion() also
+// measured faster with it (FP64: 4% slower).  Separate kernels for the two
+// cases: one kernel holding both bodies measured up to 20% slower (FP32).
+template <typename T>
+inline bool pass_nt(const PassParams<T>& p) {
+  return p.stream || sizeof(T) == 4;
 }
 
 // Padded length class for an axis of length M (0 if unsupported).
@@ -663,15 +689,15 @@ struct SlabParams {
   const unsigned short* lo_y;
   int X, Y, Z;
   int shift;       // centrosymmetric pairing (see cen_partner)
+  int stream;      // 1: non-temporal field traffic (field_load)
 };
 
-template <typename T, int MP, bool CEN>
-__global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
+template <typename T, int MP, bool CEN, bool NTM>
+__device__ __forceinline__ void slab_zy_body(const SlabParams<T>& p, T* lds) {
   using S = SlabShape<MP, CEN>;
   constexpr int NT = S::NT, P = S::P, JT = S::JT, TPW = S::TPW;
   constexpr int H = MP / 2;
   using V4 = typename MfmaOp<T>::v4;
-  __shared__ T lds[MP * P];
 
   const PassChain<T> ch = p.c[blockIdx.y];
   const int Y = p.Y, Z = p.Z;
@@ -698,7 +724,7 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
       gy[k] = f / MP;
       gz[k] = f - (f / MP) * MP;
       ok[k] = f < MP * MP && gy[k] < Y && gz[k] < Z;
-      v[k] = ok[k] ? ch.in[base + (long)gy[k] * Z + gz[k]] : T(0);
+      v[k] = ok[k] ? field_load<NTM>(&ch.in[base + (long)gy[k] * Z + gz[k]]) : T(0);
     }
     if (ch.pre) {
       T e[KE];
@@ -721,8 +747,8 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
       const int y = f / H, z = f - (f / H) * H;
       g1[k] = base + (long)y * MP + z;
       g2[k] = base + (long)y * MP + cen_partner<MP>(z, shift);
-      x1[k] = ch.in[g1[k]];
-      x2[k] = ch.in[g2[k]];
+      x1[k] = field_load<NTM>(&ch.in[g1[k]]);
+      x2[k] = field_load<NTM>(&ch.in[g2[k]]);
     }
     if (ch.pre) {
       T e1[KE], e2[KE];
@@ -823,7 +849,7 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
         const long g = base + (long)y * Z + z;
         T v = lds[y * P + z];
         if (ch.post) v *= ch.post[g];
-        ch.out[g] = v;
+        field_store<NTM>(v, &ch.out[g]);
       }
     }
   } else {
@@ -840,10 +866,25 @@ __global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
         o1 *= ch.post[g1];
         o2 *= ch.post[g2];
       }
-      ch.out[g1] = o1;
-      ch.out[g2] = o2;
+      field_store<NTM>(o1, &ch.out[g1]);
+      field_store<NTM>(o2, &ch.out[g2]);
     }
   }
+}
+
+template <typename T, int MP, bool CEN, bool NTM>
+__global__ __launch_bounds__(512) void slab_zy_kernel(SlabParams<T> p) {
+  __shared__ T lds[MP * SlabShape<MP, CEN>::P];
+  slab_zy_body<T, MP, CEN, NTM>(p, lds);
+}
+
+// Non-temporal field traffic in the slab when the caller streams fields (a
+// propagation): measured 2-8% faster per step for FP64 and 5-7% for the FP32
+// half-size products; dense FP32 products measured 11% slower with it (class
+// 48) and keep ordinary accesses, as does a lone diffusion().
+template <typename T>
+inline bool slab_nt(const SlabParams<T>& p, bool cen) {
+  return p.stream && (cen || sizeof(T) == 8);
 }
 
 bool slab_enabled();
@@ -859,25 +900,50 @@ inline bool slab_applies(int Y, int Z) {
 // hipErrorNotSupported otherwise (the caller then runs the two passes).
 // `cen` selects the centrosymmetric kernel; the caller passes the half-size
 // operators in that case and guarantees Y == Z == MP with MP % 32 == 0.
+template <typename T, int MP, bool CEN>
+void launch_slab_mp(const SlabParams<T>& p, dim3 grid, bool nt, hipStream_t s) {
+  if constexpr (CEN || sizeof(T) == 8) {
+    if (nt) {
+      hipLaunchKernelGGL((slab_zy_kernel<T, MP, CEN, true>), grid, dim3(512), 0, s, p);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((slab_zy_kernel<T, MP, CEN, false>), grid, dim3(512), 0, s, p);
+}
+
 template <typename T>
 hipError_t launch_slab_zy(const SlabParams<T>& p, int nchains, bool cen, hipStream_t s) {
   const int cy = pass_class(p.Y), cz = pass_class(p.Z);
   if (!slab_enabled() || cy != cz || cy > 64 || cy == 0 || nchains < 1 || nchains > 2)
     return hipErrorNotSupported;
   dim3 grid((unsigned)p.X, (unsigned)nchains, 1);
+  const bool nt = slab_nt(p, cen);
   if (cen) {
     if (p.Y != cy || p.Z != cz || cy % 32 != 0) return hipErrorInvalidValue;
-    if (cy == 32) hipLaunchKernelGGL((slab_zy_kernel<T, 32, true>), grid, dim3(512), 0, s, p);
-    else hipLaunchKernelGGL((slab_zy_kernel<T, 64, true>), grid, dim3(512), 0, s, p);
+    if (cy == 32) launch_slab_mp<T, 32, true>(p, grid, nt, s);
+    else launch_slab_mp<T, 64, true>(p, grid, nt, s);
     return hipGetLastError();
   }
   switch (cy) {
-    case 16: hipLaunchKernelGGL((slab_zy_kernel<T, 16, false>), grid, dim3(512), 0, s, p); break;
-    case 32: hipLaunchKernelGGL((slab_zy_kernel<T, 32, false>), grid, dim3(512), 0, s, p); break;
-    case 48: hipLaunchKernelGGL((slab_zy_kernel<T, 48, false>), grid, dim3(512), 0, s, p); break;
-    default: hipLaunchKernelGGL((slab_zy_kernel<T, 64, false>), grid, dim3(512), 0, s, p); break;
+    case 16: launch_slab_mp<T, 16, false>(p, grid, nt, s); break;
+    case 32: launch_slab_mp<T, 32, false>(p, grid, nt, s); break;
+    case 48: launch_slab_mp<T, 48, false>(p, grid, nt, s); break;
+    default: launch_slab_mp<T, 64, false>(p, grid, nt, s); break;
   }
   return hipGetLastError();
+}
+
+template <typename T, int AXIS, int MP, bool CEN>
+void launch_mfma_pass(const PassParams<T>& p, dim3 grid, hipStream_t s) {
+  if constexpr (sizeof(T) == 8) {
+    if (!pass_nt(p)) {
+      hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP, CEN, false>), grid,
+                         dim3(MfmaShape<MP>::NT), 0, s, p);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP, CEN, true>), grid,
+                     dim3(MfmaShape<MP>::NT), 0, s, p);
 }
 
 template <typename T, int AXIS, int MP>
@@ -892,13 +958,12 @@ hipError_t launch_axis_pass_mp(const PassParams<T>& p, int nchains, hipStream_t 
       if (p.cen) {
         if constexpr (MP % 32 == 0) {
           if (M != MP) return hipErrorInvalidValue;
-          hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP, true>), grid, dim3(S::NT), 0, s,
-                             p);
+          launch_mfma_pass<T, AXIS, MP, true>(p, grid, s);
           return hipGetLastError();
         }
         return hipErrorInvalidValue;
       }
-      hipLaunchKernelGGL((axis_pass_mfma_kernel<T, AXIS, MP>), grid, dim3(S::NT), 0, s, p);
+      launch_mfma_pass<T, AXIS, MP, false>(p, grid, s);
       return hipGetLastError();
     }
   }

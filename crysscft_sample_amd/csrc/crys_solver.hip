@@ -274,18 +274,22 @@ int build_ops(crys_solver* h, double ds, OpSet* out) {
 // One contour step E D E (or a bare D with E == nullptr) for 1 or 2 chains:
 // z pass in -> scr (pre E), y pass in place on scr, x pass scr -> out (post E).
 //
+// `stream` marks a step of a propagation, whose fields are read once by the
+// next launch: the matrix-core passes and slab then read and write them
+// non-temporally.  A lone diffusion() passes false (its field stays in L2).
 // `only` >= 0 issues only that launch of the step (0-based, in issue order);
 // *nlaunch (if not null) receives the number of launches of a whole step.
 template <typename T>
 int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const in[2],
                 T* const out[2], T* const scr[2], const T* const E[2], hipStream_t s,
-                int only = -1, int* nlaunch = nullptr) {
+                bool stream, int only = -1, int* nlaunch = nullptr) {
   const int X = (int)h->nphy[0], Y = (int)h->nphy[1], Z = (int)h->nphy[2];
   crys::PassParams<T> p{};
   p.X = X;
   p.Y = Y;
   p.Z = Z;
   p.shift = h->kind == CRYS_3M;
+  p.stream = stream;
   crys::RadixParams<T> rp{};
   rp.X = X;
   rp.Y = Y;
@@ -327,6 +331,7 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
     sp.Y = Y;
     sp.Z = Z;
     sp.shift = h->kind == CRYS_3M;
+    sp.stream = stream;
     const bool cen_zy = op.dmat_c[1] && op.dmat_c[2] && Y == Z;
     for (int c = 0; c < nch; ++c) sp.c[c] = {in[c], scr[c], E[c], nullptr};
     sp.mat_z = (const T*)(cen_zy ? op.dmat_c[2] : op.dmat[2]);
@@ -458,7 +463,7 @@ int diffusion_impl(crys_solver* h) {
     double* out[2] = {io, nullptr};
     double* scr[2] = {io, nullptr};
     const double* E[2] = {nullptr, nullptr};
-    return launch_step<double>(h, op, 1, in, out, scr, E, h->stream);
+    return launch_step<double>(h, op, 1, in, out, scr, E, h->stream, false);
   }
   float* w = (float*)h->work;
   hipLaunchKernelGGL((crys::convert_kernel<float, double>), dim3(grid_for(n)), dim3(256), 0,
@@ -468,7 +473,7 @@ int diffusion_impl(crys_solver* h) {
   float* out[2] = {w, nullptr};
   float* scr[2] = {w, nullptr};
   const float* E[2] = {nullptr, nullptr};
-  CHK(launch_step<float>(h, op, 1, in, out, scr, E, h->stream));
+  CHK(launch_step<float>(h, op, 1, in, out, scr, E, h->stream, false));
   hipLaunchKernelGGL((crys::convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0,
                      h->stream, h->iomat, (const float*)w, n);
   HIPCHK(hipGetLastError());
@@ -808,7 +813,7 @@ int crys_propagate(crys_solver_t* h, const double* E, const double* init, size_t
     double* out[2] = {q + sn * n, nullptr};
     double* sc[2] = {scr, nullptr};
     const double* Ee[2] = {E, nullptr};
-    CHK(launch_step<double>(h, op, 1, in, out, sc, Ee, h->stream));
+    CHK(launch_step<double>(h, op, 1, in, out, sc, Ee, h->stream, true));
     s = sn;
   }
   HIPCHK(hipMemcpyAsync(h->iomat, q + s * n, n * sizeof(double), hipMemcpyDeviceToDevice,
@@ -857,7 +862,7 @@ int propagate_segment(crys_scft* h, hipStream_t s, int nch, T* qf, T* qb, size_t
       const T* in[2] = {qf + t * n, qb + (c - t) * n};
       T* out[2] = {qf + (t + 1) * n, qb + bo * n};
       const T* E[2] = {t0 + t < NsA ? EA : EB, t0 + bo >= NsA ? EB : EA};
-      CHK(launch_step<T>(sv, op, nch, in, out, scr, E, s));
+      CHK(launch_step<T>(sv, op, nch, in, out, scr, E, s, true));
     }
   }
   return CRYS_OK;
@@ -1496,7 +1501,7 @@ int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* m
       float* out[2] = {qf + n, qb + (Ns - 1) * n};
       float* sc[2] = {(float*)h->scr[0], (float*)h->scr[1]};
       const float* E[2] = {(const float*)h->EA, (const float*)h->EB};
-      return launch_step<float>(sv, op, 2, in, out, sc, E, s, only, &nl);
+      return launch_step<float>(sv, op, 2, in, out, sc, E, s, true, only, &nl);
     }
     double* qf = (double*)h->qf;
     double* qb = (double*)h->qb;
@@ -1504,7 +1509,7 @@ int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* m
     double* out[2] = {qf + n, qb + (Ns - 1) * n};
     double* sc[2] = {(double*)h->scr[0], (double*)h->scr[1]};
     const double* E[2] = {(const double*)h->EA, (const double*)h->EB};
-    return launch_step<double>(sv, op, 2, in, out, sc, E, s, only, &nl);
+    return launch_step<double>(sv, op, 2, in, out, sc, E, s, true, only, &nl);
   };
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
