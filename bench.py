@@ -360,7 +360,7 @@ def main():
                     help="time COUNT batched instances (the batching workload) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not run "
               "(the path would shard by independent SCFT instances, one batch per GPU)", file=sys.stderr)
         return 2
     if args.batch:

@@ -53,7 +53,7 @@ enum crys_status {
   CRYS_ERR_INVALID_ARGUMENT = 1, /* NULL pointer, bad direction, ds <= 0, ... */
   CRYS_ERR_BAD_GRID = 2,         /* the grid the reference's set_grid() rejects */
   CRYS_ERR_NO_CONTOUR_STEP = 3,  /* diffusion before set_contour_step */
-  CRYS_ERR_UNSUPPORTED = 4,      /* a combination this build does not run */
+  CRYS_ERR_UNSUPPORTED = 4,      /* a combination the library does not run */
   CRYS_ERR_HIP = 5,              /* a HIP runtime call failed */
   CRYS_ERR_OUT_OF_MEMORY = 6,
   CRYS_ERR_DIVERGED = 7,         /* non-finite free energy or Q */
