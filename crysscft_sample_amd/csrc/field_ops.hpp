@@ -154,11 +154,13 @@ __global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
     // sum_s w_s q[s] q+[s] over slots s0..s1 (trapezoid: the end slots
     // weigh 1/2); R slots per round with independent partial sums, so that
     // 2 R loads are in flight per point (the grid holds 8 waves per CU):
-    // R = 8 for FP32 (measured 91 -> 77 us at 128^3), 4 for FP64 (8 was
-    // slower).  Each history entry is read once here, non-temporally: the
-    // FP64 iteration at 128^3 measured 1.7% faster than with ordinary loads.
+    // R = 8 (FP32 measured 91 -> 77 us at 128^3 against R = 4).  Each
+    // history entry is read once here, non-temporally: the FP64 iteration at
+    // 128^3 measured 1.7% faster than with ordinary loads, and with them R = 8
+    // also beats R = 4 for FP64 (0.3% per iteration; slower with ordinary
+    // loads).
     auto contour_sum = [&](size_t s0, size_t s1) -> double {
-      constexpr int R = sizeof(T) == 4 ? 8 : 4;
+      constexpr int R = 8;
       double p[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) p[r] = 0.0;
