@@ -18,6 +18,17 @@
  *   - Driver: uniform-field closed form Q = exp(-ds(a NsA + b NsB)) with
  *     phiA == fA, phiB == fB; mean(phiA) = fA and mean(phiB) = fB for any
  *     field; d lnQ / dwA(r) = -(ds NsA / fA) phiA(r) / n by finite differences.
+ *   - print_energy terms: closed forms on uniform fields (enthalpic xN fA fB,
+ *     -ln Q, entropic -(a fA + b fB), ksi term 0, MaxIncompressibility 0),
+ *     truncated NsA included; for any fields the identity
+ *     F - H = <xN t^2/4> - <(v + xN d)^2/(4 xN)> with the density-free
+ *     H = <(wA-wB)^2/(4 xN) - (wA+wB)/2 + xN/4> - ln Q, t = 1-phiA-phiB,
+ *     d = phiA-phiB, v = wA-wB; F = H at the fixed point of the mixing.
+ *   - calc_new_fields: closed form on uniform fields (pre-update ksi); for any
+ *     fields wA+wB moves by -acc xN t and wA-wB by -acc (v + xN d).
+ *     A copy of this file with a sign flipped in any print_energy term, the
+ *     densities swapped in a term, or the wrong density in the mixing fails
+ *     these checks.
  *   - 3m with non-zero translational parts: the phase-folded 3m equals P1
  *     applied to the field expanded by the 8 symmetry operations, then
  *     reduced to the even sub-lattice (6 translation sets, two of them with

@@ -265,3 +265,106 @@ def test_scft_golden_traces_agree_across_solvers(oracle):
     s = oracle.Solver("PmmmDCT", (N,) * 3)
     tr, *_ = s.scft(fA, ds, xN, L, 0.1, 1, F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT"))
     assert np.abs(tr[0] - p[0]).max() <= 1e-13 * np.abs(p[0]).max()
+
+
+# ---- print_energy and calc_new_fields -------------------------------------
+# print_energy (crysSCFT_sample.cpp:398-424) and calc_new_fields (:362-371)
+# are pinned by what the mathematics fixes, not by their own formulas:
+#   * uniform fields wA = a, wB = b: every trace column and the mixed fields
+#     in closed form (q(s) q+(s) = Q at every s, so phiA = fA, phiB = fB);
+#   * any fields: with H = <(wA-wB)^2/(4 xN) - (wA+wB)/2 + xN/4> - ln Q (no
+#     density in it), t = 1-phiA-phiB, d = phiA-phiB and v = wA-wB, the four
+#     printed terms satisfy the algebraic identity
+#         F - H = <xN t^2 / 4> - <(v + xN d)^2 / (4 xN)>,
+#     which a dropped or sign-flipped term, or phiA/phiB swapped in a term,
+#     violates by O(the term);
+#   * the mixing moves wA + wB by -acc xN t and wA - wB by -acc (v + xN d);
+#   * at the fixed point of the mixing the printed F equals H built from Q.
+def _uniform_case(oracle, name, fA, ds, a, b, xN, acc):
+    s = oracle.Solver(name, (8, 8, 16))
+    shape = s.grid_size_phy
+    return s.scft(fA, ds, xN, 4.0, acc, 1, np.full(shape, a), np.full(shape, b))
+
+
+@pytest.mark.parametrize("name", ["P1", "3m", "PmmmDCT"])
+@pytest.mark.parametrize("fA,ds", [(0.3, 0.01), (0.28, 1 / 128)])  # NsA = 30, 35 (truncated)
+def test_print_energy_uniform_closed_form(oracle, name, fA, ds):
+    xN, a, b, acc = 17.0, 0.7, -0.4, 0.15
+    tr, wA, wB, phiA, phiB = _uniform_case(oracle, name, fA, ds, a, b, xN, acc)
+    fB = 1 - fA
+    NsA, NsB = int(fA / ds), int(fB / ds)
+    enth, mlnq, entr = xN * fA * fB, ds * (a * NsA + b * NsB), -(a * fA + b * fB)
+    r = tr[0]
+    assert abs(r[1] - enth) < 1e-13 * abs(enth)
+    assert abs(r[2] - mlnq) < 1e-12 * abs(mlnq)
+    assert abs(r[3] - entr) < 1e-12 * abs(entr)
+    assert abs(r[4]) < 1e-13
+    assert r[5] < 1e-13
+    assert abs(r[0] - (enth + mlnq + entr)) < 1e-12 * abs(r[0])
+    assert abs(r[6] - np.exp(-mlnq)) < 1e-13 * np.exp(-mlnq)
+    # calc_new_fields with the pre-update ksi = (a + b - xN) / 2 (calc_ksi, :341-346)
+    ksi = 0.5 * (a + b - xN)
+    assert np.abs(wA - (a + acc * (xN * fB + ksi - a))).max() < 1e-13
+    assert np.abs(wB - (b + acc * (xN * fA + ksi - b))).max() < 1e-13
+
+
+def _noisy_fields(name, N, fA, xN, L, seed=3, amp=0.5):
+    wA, wB = F.bcc_fields((N,) * 3, L, fA, xN)
+    a, b = F.extract(wA, name), F.extract(wB, name)
+    rng = np.random.default_rng(seed)
+    return a + amp * rng.standard_normal(a.shape), b + amp * rng.standard_normal(b.shape)
+
+
+def field_H(wA, wB, xN, Q):
+    """The field-theoretic free energy: no density enters it."""
+    return float(np.mean((wA - wB) ** 2 / (4 * xN) - (wA + wB) / 2 + xN / 4) - np.log(Q))
+
+
+@pytest.mark.parametrize("name", ["P1", "3m", "PmmmDCT"])
+def test_print_energy_identity_any_fields(oracle, name):
+    N, fA, xN, L, ds = 16, 0.28, 22.0, 4.0, 1 / 64
+    a, b = _noisy_fields(name, N, fA, xN, L)
+    s = oracle.Solver(name, (N,) * 3)
+    tr, wA, wB, phiA, phiB = s.scft(fA, ds, xN, L, 0.0, 1, a, b)
+    r = tr[0]
+    np.testing.assert_array_equal(wA, a)  # acceptance 0 leaves the fields
+    t, d, v = 1 - phiA - phiB, phiA - phiB, a - b
+    rhs = float(np.mean(xN * t ** 2 / 4) - np.mean((v + xN * d) ** 2 / (4 * xN)))
+    assert abs((r[0] - field_H(a, b, xN, r[6])) - rhs) < 1e-13 * abs(r[0])
+    # every term is far above that tolerance, so dropping or flipping any fails
+    for k in (1, 2, 3, 4):
+        assert abs(r[k]) > 1e-6
+    assert r[0] == r[1] + r[2] + r[3] + r[4]
+    assert r[2] == -np.log(r[6])
+    assert abs(r[5] - np.abs(t).max()) < 1e-15
+    assert r[5] > 1e-3
+
+
+@pytest.mark.parametrize("name", ["P1", "PmmmDCT"])
+def test_mixing_moves_the_fields_as_the_densities_fix(oracle, name):
+    N, fA, xN, L, ds, acc = 16, 0.28, 22.0, 4.0, 1 / 64, 0.2
+    a, b = _noisy_fields(name, N, fA, xN, L, seed=5)
+    s = oracle.Solver(name, (N,) * 3)
+    _, wA, wB, phiA, phiB = s.scft(fA, ds, xN, L, acc, 1, a, b)
+    t, d, v = 1 - phiA - phiB, phiA - phiB, a - b
+    scale = np.abs(a).max()
+    assert np.abs((wA + wB) - (a + b) + acc * xN * t).max() < 1e-13 * scale
+    assert np.abs((wA - wB) - v + acc * (v + xN * d)).max() < 1e-13 * scale
+
+
+def test_converged_free_energy_equals_field_functional(oracle):
+    """At the fixed point of simple mixing (wA = xN phiB + ksi, wB = xN phiA +
+    ksi, hence phiA + phiB = 1) the printed F equals H[w], which is built from
+    the fields and Q alone: an independent route to F."""
+    N, fA, xN, L, ds, acc = 16, 0.25, 20.0, 4.0, 0.02, 0.5
+    wA, wB = F.bcc_fields((N,) * 3, L, fA, xN)
+    a, b = F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT")
+    s = oracle.Solver("PmmmDCT", (N,) * 3)
+    tr, wA2, wB2, _, _ = s.scft(fA, ds, xN, L, acc, 600, a, b)
+    assert tr[-1, 5] < 1e-10
+    # the last iteration's fields: one more evaluation without mixing
+    tr1, *_ = s.scft(fA, ds, xN, L, 0.0, 1, wA2, wB2)
+    F_last, Q_last = tr1[0, 0], tr1[0, 6]
+    assert abs(F_last - field_H(wA2, wB2, xN, Q_last)) < 1e-11 * abs(F_last)
+    # and the structured phase is not the disordered melt
+    assert abs(F_last - (xN * fA * (1 - fA))) > 1e-3
