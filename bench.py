@@ -15,8 +15,10 @@ propagation sequence, captured in a hipGraph and timed with HIP events on the
 solver's stream (crys_scft_time_propagation); the same kernel launched alone
 and repeated is reported beside it.  "contour_step" gives the whole step
 against the step's minimum bytes.  The CPU baseline is the oracle (a C
-restatement of the reference) timed on this host on one SCFT iteration of
-the same workload.
+restatement of the reference) timed on this host on three SCFT iterations of
+the same workload on the process's CPU share, one iteration on one thread,
+and the reference's triplet-only clock for both; the first iterations of
+both GPU legs are compared with the oracle's.
 """
 from __future__ import annotations
 
@@ -50,16 +52,32 @@ STEP_KERNELS = {
         "crys::axis_pass_mfma_kernel<double,1,64,true,true>",
         "crys::axis_pass_mfma_kernel<double,0,64,true,true>"],
 }
-# HBM bytes per dispatch of both chains, from the rocprofv3 FETCH_SIZE and
-# WRITE_SIZE passes of this workload (FETCH_SIZE scaled by 1.975 from a copy
-# of known size), profiles/hbm_traffic_plane_phase_pmmm128_fp64.md.
-MEASURED_TRAFFIC = {
-    "crys::plane_phase_half_kernel<double>": 16.18e6,
-    "crys::plane_phase_kernel<double,64,true,1024>": 16.17e6,
-    # measured before these two kernels read and wrote non-temporally
-    "crys::slab_zy_kernel<double,64,true,true>": 12.00e6,
-    "crys::axis_pass_mfma_kernel<double,0,64,true,true>": 11.86e6,
-}
+# HBM bytes per dispatch of both chains come from the rocprofv3 FETCH_SIZE
+# and WRITE_SIZE passes of this workload (FETCH_SIZE scaled by its reading on
+# a copy of known size), kept in this profile summary: the row of a kernel
+# gives its calibrated MB per dispatch.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_plane_phase_pmmm128_fp64.md")
+
+
+def measured_traffic(kernel):
+    """(bytes per dispatch, profile row name) of `kernel` from TRAFFIC_PROFILE,
+    or (None, None) when the profile has no row for it."""
+    try:
+        lines = open(TRAFFIC_PROFILE).read().splitlines()
+    except OSError:
+        return None, None
+    want = kernel.replace(" ", "")
+    for ln in lines:
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) < 6 or not cells[0].startswith("`"):
+            continue
+        name = cells[0].split("`")[1]
+        if name.replace(" ", "") == want:
+            try:
+                return float(cells[4]) * 1e6, name
+            except ValueError:
+                return None, None
+    return None, None
 
 
 def bytes_per_iteration(n, b, NsA, NsB):
@@ -144,16 +162,40 @@ def checkpoint_timing(steps):
     return 0
 
 
-def cpu_baseline(wA, wB, threads):
-    """The oracle on one SCFT iteration of the same workload (bounded sample)."""
+def cpu_threads():
+    """Threads of the CPU baseline: the process's CPU share (OMP_NUM_THREADS
+    when set, else the CPUs this process may run on)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(wA, wB, threads, iters):
+    """The oracle on `iters` SCFT iterations of the same workload from the
+    same fields (a bounded sample), on `threads` OpenMP threads: iterations/s
+    of the whole iteration, and the reference's own clock, the time spent in
+    diffusion() alone (std::clock around solver.diffusion(),
+    crysSCFT_sample.cpp:389-391), from timed triplets on the same solver."""
     from oracle import oracle as O
 
     O.set_num_threads(threads)
     s = O.Solver("PmmmDCT", (N_LOG,) * 3)
     t0 = time.perf_counter()
-    tr, *_ = s.scft(FA, DS, XN, L, ACC, 1, wA, wB)
+    tr, *_ = s.scft(FA, DS, XN, L, ACC, iters, wA, wB)
     dt = time.perf_counter() - t0
-    return 1.0 / dt, tr
+    io = s.iomat()
+    io[...] = 1.0 + 0.01 * np.asarray(wA).reshape(io.shape)
+    reps = 20
+    s.diffusion()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        s.diffusion()
+    triplet_ms = (time.perf_counter() - t1) / reps * 1e3
+    return iters / dt, tr, triplet_ms
 
 
 def pmc_workload():
@@ -406,7 +448,9 @@ def main():
     kernel = names[k_dom] if k_dom < len(names) else f"launch {k_dom}"
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": MEASURED_TRAFFIC.get(kernel),
+                "traffic": measured_traffic(kernel)[0],
+                "traffic_source": (os.path.relpath(TRAFFIC_PROFILE, ROOT)
+                                   if measured_traffic(kernel)[0] else None),
                 "kernel": kernel + " (both chains)",
                 "ms_per_launch": round(t_dom, 6),
                 "ms_per_launch_repeated_alone": round(t_alone, 6),
@@ -433,16 +477,32 @@ def main():
             "achieved_gbs": round(step_bytes / (f64["ms_per_step_in_sequence"] * 1e-3) / 1e9, 2)}
 
     cpu = None
+    tr_cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        print(f"bench: CPU baseline (oracle, {threads} threads, 1 iteration)", file=sys.stderr,
-              flush=True)
-        v, tr_cpu = cpu_baseline(wA, wB, threads)
-        dF = abs(f64["warm"][0, 0] / tr_cpu[0, 0] - 1)
-        cpu = {"value": round(v, 4), "unit": "iter/s", "cores": threads, "kind": "port",
-               "sample": "1 SCFT iteration (400 PmmmDCT diffusions + driver) of the same "
-                         "128^3 Ns=200 workload, own radix DCT, OpenMP over lines",
-               "iter0_rel_dF_vs_gpu": dF}
+        threads = cpu_threads()
+        iters = 3
+        print(f"bench: CPU baseline (oracle, {threads} threads, {iters} iterations; then 1 thread)",
+              file=sys.stderr, flush=True)
+        v, tr_cpu, trip = cpu_baseline(wA, wB, threads, iters)
+        v1, tr1, trip1 = cpu_baseline(wA, wB, 1, 1)
+        O_threads = threads
+        dF = float(np.abs(f64["warm"][:iters, 0] / tr_cpu[:, 0] - 1).max())
+        # the reference's own figure: triplet milliseconds, 2 (NsA + NsB) = 400
+        # triplets per iteration (crysSCFT_sample.cpp:389-391, :512)
+        cpu = {"value": round(v, 4), "unit": "iter/s", "cores": O_threads, "kind": "port",
+               "sample": f"{iters} SCFT iterations (400 PmmmDCT diffusions + driver each) of the "
+                         "same 128^3 Ns=200 workload from the same fields, own radix DCT, OpenMP "
+                         f"over lines; 1 iteration on 1 thread",
+               "host_cpus": os.cpu_count(),
+               "value_1thread": round(v1, 5),
+               "triplet_ms": round(trip, 3),
+               "triplet_ms_1thread": round(trip1, 3),
+               "triplet_s_per_iter": round(trip * 2 * (NsA + NsB) / 1e3, 4),
+               "triplet_s_per_iter_1thread": round(trip1 * 2 * (NsA + NsB) / 1e3, 4),
+               "gpu_fp64_rel_dF_vs_oracle": dF,
+               "gpu_fp64_rel_dMaxInc_vs_oracle": float(np.abs(
+                   f64["warm"][:iters, 5] / tr_cpu[:, 5] - 1).max()),
+               "oracle_1thread_rel_dF": float(abs(tr1[0, 0] / tr_cpu[0, 0] - 1))}
 
     line = {
         "metric": "SCFT iter/s at 128^3 Ns=200 PmmmDCT: mixed vs FP64 GPU; HBM GB/s",
@@ -478,6 +538,11 @@ def main():
                              mixed["trace"][:, 5] / f64["trace"][:, 5] - 1).max()),
                          "max_abs_dphiA_vs_fp64_gpu": float(np.abs(mixed["phiA"] - f64["phiA"]).max()),
                          "max_abs_dphiB_vs_fp64_gpu": float(np.abs(mixed["phiB"] - f64["phiB"]).max()),
+                         "max_rel_dF_vs_oracle": (None if tr_cpu is None else float(np.abs(
+                             mixed["warm"][:len(tr_cpu), 0] / tr_cpu[:, 0] - 1).max())),
+                         "max_rel_dMaxInc_vs_oracle": (None if tr_cpu is None else float(np.abs(
+                             mixed["warm"][:len(tr_cpu), 5] / tr_cpu[:, 5] - 1).max())),
+                         "oracle_iterations_compared": None if tr_cpu is None else len(tr_cpu),
                          "ms_per_contour_step": round(mixed["ms_per_step_in_sequence"], 6),
                          "mfma_fp32_frac": round(flops_launch / (mixed["ms_per_step_in_sequence"] * 1e-3)
                                                  / 1e12 / MFMA_PEAK_TFLOPS["fp32"], 5),

@@ -113,6 +113,8 @@ SIGNATURES = {
                                            ctypes.POINTER(_c_sz), ctypes.POINTER(_c_sz)]),
     "crys_scft_evaluate": (ctypes.c_int, [_c_vp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp,
                                           _c_dp]),
+    "crys_scft_evaluate_device": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_dp, _c_vp, _c_vp, _c_dp,
+                                                 _c_dp]),
 }
 
 _lib = None

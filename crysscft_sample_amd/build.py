@@ -59,6 +59,26 @@ def build_library(verbose: bool = True) -> str:
     return LIB_PATH
 
 
+# The reference's solver plugin header (solver_base.h and utilities/c_array.h),
+# read where the reference lies; only this container has it.
+REFERENCE_SRC = "/root/reference/src"
+ADAPTER_DIR = os.path.join(PKG, "integration")
+ADAPTER_CHECK = os.path.join(ADAPTER_DIR, "adapter_check")
+
+
+def build_adapter_check(out: str = ADAPTER_CHECK) -> str | None:
+    """Compile integration/adapter_check.cpp, the C++ adapter solver_gpu.h
+    driven through the reference's own CrysPSSolverBase, against
+    solver_base.h from the reference sources and libcrys_solver.so.  Returns
+    the binary, or None when the reference sources are absent."""
+    if not os.path.exists(os.path.join(REFERENCE_SRC, "solver_base.h")):
+        return None
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I", REFERENCE_SRC,
+          "-I", INCLUDE, "-I", ADAPTER_DIR, os.path.join(ADAPTER_DIR, "adapter_check.cpp"),
+          "-L", PKG, "-lcrys_solver", "-Wl,-rpath,$ORIGIN/..", "-o", out])
+    return out
+
+
 def build_oracle() -> str:
     oracle_dir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-B", "-C", oracle_dir, "liboracle.so"])
@@ -68,6 +88,7 @@ def build_oracle() -> str:
 def build_all() -> None:
     build_library()
     build_oracle()
+    build_adapter_check()
 
 
 if __name__ == "__main__":

@@ -145,6 +145,9 @@ struct crys_solver {
   void* scratch = nullptr;  // device, n_phy elements of the precision
   std::vector<OpSet> ops;
   int cur = -1;
+  // bumped whenever the operator set in use changes (new cell, new or other
+  // cached ds): a captured iteration graph records the operators' addresses
+  unsigned long ops_gen = 0;
   // 3m with non-zero translational parts: diffusion() expands the reduced
   // field to the logical grid, applies the P1 operators of `full` (a P1
   // solver on this solver's stream) and extracts the even sub-lattice
@@ -158,6 +161,7 @@ struct crys_scft {
   // a 3m solver with translational parts: the iteration runs on sv (its P1
   // solver) with the fields expanded; m3 is the 3m solver the caller passed
   const crys_solver* m3 = nullptr;
+  crys_solver* user = nullptr;  // the solver the caller passed (sv, or the 3m solver)
   double fA = 0, fB = 0, ds = 0, xN = 0, acc = 0, L[3] = {0, 0, 0};
   size_t NsA = 0, NsB = 0, Ns = 0, n = 0;
   double *wA = nullptr, *wB = nullptr, *ksi = nullptr, *phiA = nullptr, *phiB = nullptr;
@@ -174,7 +178,7 @@ struct crys_scft {
   void* ck = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  const void* graph_mats[3] = {nullptr, nullptr, nullptr};
+  unsigned long graph_gen = 0;  // sv->ops_gen when the graph was captured
   bool fields_set = false;
 };
 
@@ -194,6 +198,7 @@ void free_ops(crys_solver* h) {
     }
   h->ops.clear();
   h->cur = -1;
+  ++h->ops_gen;
 }
 
 
@@ -450,7 +455,7 @@ int diffusion_impl(crys_solver* h) {
     HIPCHK(hipGetLastError());
     CHK(diffusion_impl(h->full));
     hipLaunchKernelGGL(crys::m3_extract_kernel, dim3(grid_for(h->n_phy)), dim3(256), 0, h->stream,
-                       (const double*)h->full->iomat, h->iomat, X, Y, Z);
+                       (const double*)h->full->iomat, h->iomat, X, Y, Z, 1.0);
     HIPCHK(hipGetLastError());
     return CRYS_OK;
   }
@@ -477,6 +482,61 @@ int diffusion_impl(crys_solver* h) {
   hipLaunchKernelGGL((crys::convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0,
                      h->stream, h->iomat, (const float*)w, n);
   HIPCHK(hipGetLastError());
+  return CRYS_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// solve_diffusion_equation on device buffers; `scr` (n doubles) stages the
+// half-finished step between the passes.
+int propagate_impl(crys_solver_t* h, const double* E, const double* init, size_t Ns,
+                   int direction, double* q, double* scr) {
+  CHK(check_handle(h));
+  if (!E || !init || !q) return fail(CRYS_ERR_INVALID_ARGUMENT, "null device buffer");
+  if (direction != 1 && direction != -1)
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "Wrong direction");  // crysSCFT_sample.cpp:381
+  if (h->precision != CRYS_FP64)
+    return fail(CRYS_ERR_UNSUPPORTED, "crys_propagate takes FP64 histories; use an FP64 solver");
+  const size_t n = h->n_phy;
+  if (h->full) {
+    // 3m with translational parts: E, diffusion() on the iomat, E per step
+    if (h->full->cur < 0)
+      return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
+    size_t s = direction == 1 ? 0 : Ns;
+    HIPCHK(hipMemcpyAsync(q + s * n, init, n * sizeof(double), hipMemcpyDeviceToDevice,
+                          h->stream));
+    for (size_t it = 0; it < Ns; ++it) {
+      const size_t sn = direction == 1 ? s + 1 : s - 1;
+      hipLaunchKernelGGL(crys::mul_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                         (const double*)(q + s * n), E, h->iomat, n);
+      HIPCHK(hipGetLastError());
+      CHK(diffusion_impl(h));
+      hipLaunchKernelGGL(crys::mul_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                         (const double*)h->iomat, E, q + sn * n, n);
+      HIPCHK(hipGetLastError());
+      s = sn;
+    }
+    HIPCHK(hipMemcpyAsync(h->iomat, q + s * n, n * sizeof(double), hipMemcpyDeviceToDevice,
+                          h->stream));
+    return CRYS_OK;
+  }
+  if (h->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
+  const OpSet& op = h->ops[h->cur];
+  size_t s = direction == 1 ? 0 : Ns;
+  HIPCHK(hipMemcpyAsync(q + s * n, init, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  for (size_t it = 0; it < Ns; ++it) {
+    const size_t sn = direction == 1 ? s + 1 : s - 1;
+    const double* in[2] = {q + s * n, nullptr};
+    double* out[2] = {q + sn * n, nullptr};
+    double* sc[2] = {scr, nullptr};
+    const double* Ee[2] = {E, nullptr};
+    CHK(launch_step<double>(h, op, 1, in, out, sc, Ee, h->stream, true));
+    s = sn;
+  }
+  HIPCHK(hipMemcpyAsync(h->iomat, q + s * n, n * sizeof(double), hipMemcpyDeviceToDevice,
+                        h->stream));
   return CRYS_OK;
 }
 
@@ -640,6 +700,7 @@ int crys_set_contour_step(crys_solver_t* h, double ds) {
   h->ds_current = ds;
   for (size_t i = 0; i < h->ops.size(); ++i)
     if (h->ops[i].ds == ds) {
+      if (h->cur != (int)i) ++h->ops_gen;
       h->cur = (int)i;
       return CRYS_OK;
     }
@@ -657,6 +718,7 @@ int crys_set_contour_step(crys_solver_t* h, double ds) {
   }
   h->ops.push_back(o);
   h->cur = (int)h->ops.size() - 1;
+  ++h->ops_gen;
   return CRYS_OK;
 }
 
@@ -774,51 +836,7 @@ int crys_solver_get_stream(const crys_solver_t* h, void** out) {
 int crys_propagate(crys_solver_t* h, const double* E, const double* init, size_t Ns,
                    int direction, double* q) {
   CHK(check_handle(h));
-  if (!E || !init || !q) return fail(CRYS_ERR_INVALID_ARGUMENT, "null device buffer");
-  if (direction != 1 && direction != -1)
-    return fail(CRYS_ERR_INVALID_ARGUMENT, "Wrong direction");  // crysSCFT_sample.cpp:381
-  if (h->precision != CRYS_FP64)
-    return fail(CRYS_ERR_UNSUPPORTED, "crys_propagate takes FP64 histories; use an FP64 solver");
-  const size_t n = h->n_phy;
-  if (h->full) {
-    // 3m with translational parts: E, diffusion() on the iomat, E per step
-    if (h->full->cur < 0)
-      return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
-    size_t s = direction == 1 ? 0 : Ns;
-    HIPCHK(hipMemcpyAsync(q + s * n, init, n * sizeof(double), hipMemcpyDeviceToDevice,
-                          h->stream));
-    for (size_t it = 0; it < Ns; ++it) {
-      const size_t sn = direction == 1 ? s + 1 : s - 1;
-      hipLaunchKernelGGL(crys::mul_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream,
-                         (const double*)(q + s * n), E, h->iomat, n);
-      HIPCHK(hipGetLastError());
-      CHK(diffusion_impl(h));
-      hipLaunchKernelGGL(crys::mul_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream,
-                         (const double*)h->iomat, E, q + sn * n, n);
-      HIPCHK(hipGetLastError());
-      s = sn;
-    }
-    HIPCHK(hipMemcpyAsync(h->iomat, q + s * n, n * sizeof(double), hipMemcpyDeviceToDevice,
-                          h->stream));
-    return CRYS_OK;
-  }
-  if (h->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
-  const OpSet& op = h->ops[h->cur];
-  size_t s = direction == 1 ? 0 : Ns;
-  HIPCHK(hipMemcpyAsync(q + s * n, init, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-  double* scr = (double*)h->scratch;
-  for (size_t it = 0; it < Ns; ++it) {
-    const size_t sn = direction == 1 ? s + 1 : s - 1;
-    const double* in[2] = {q + s * n, nullptr};
-    double* out[2] = {q + sn * n, nullptr};
-    double* sc[2] = {scr, nullptr};
-    const double* Ee[2] = {E, nullptr};
-    CHK(launch_step<double>(h, op, 1, in, out, sc, Ee, h->stream, true));
-    s = sn;
-  }
-  HIPCHK(hipMemcpyAsync(h->iomat, q + s * n, n * sizeof(double), hipMemcpyDeviceToDevice,
-                        h->stream));
-  return CRYS_OK;
+  return propagate_impl(h, E, init, Ns, direction, q, (double*)h->scratch);
 }
 
 }  // extern "C"
@@ -1084,9 +1102,18 @@ int scft_build_graph(crys_scft* h) {
   if (e != hipSuccess) return fail(CRYS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
   h->graph = g;
   HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
-  const OpSet& op = sv->ops[sv->cur];
-  for (int a = 0; a < 3; ++a) h->graph_mats[a] = op.dmat[a];
+  h->graph_gen = sv->ops_gen;
   return CRYS_OK;
+}
+
+// The operators of the handle's own contour step.  The caller may have set
+// another ds on the solver, or a new cell (which drops its operators), since
+// crys_scft_create: E, NsA and NsB belong to the handle's ds, so the solver is
+// set back to it, on its current cell.
+int scft_ensure_ops(crys_scft* h) {
+  const crys_solver* sv = h->sv;
+  if (sv->cur >= 0 && sv->ops[sv->cur].ds == h->ds) return CRYS_OK;
+  return crys_set_contour_step(h->user, h->ds);
 }
 
 template <typename T>
@@ -1094,6 +1121,131 @@ void scft_prep(crys_scft* h) {
   hipLaunchKernelGGL((crys::prep_fields_kernel<T>), dim3(grid_for(h->n)), dim3(256), 0,
                      h->sv->stream, (const double*)h->wA, (const double*)h->wB, h->ksi, (T*)h->EA,
                      (T*)h->EB, h->xN, h->ds, h->n);
+}
+
+// Device fields in the solver's physical layout: copied (3m with
+// translational parts: expanded to the logical grid) into the handle's
+// fields, then calc_ksi / array_exp.  Asynchronous on the solver's stream.
+int scft_set_fields_device(crys_scft* h, const double* wA, const double* wB) {
+  hipStream_t s = h->sv->stream;
+  if (h->m3) {
+    const int X = (int)h->m3->nphy[0], Y = (int)h->m3->nphy[1], Z = (int)h->m3->nphy[2];
+    const unsigned g = grid_for(h->m3->n_phy);
+    hipLaunchKernelGGL(crys::m3_expand_kernel, dim3(g), dim3(256), 0, s, wA, h->wA, h->m3->m3, X,
+                       Y, Z);
+    hipLaunchKernelGGL(crys::m3_expand_kernel, dim3(g), dim3(256), 0, s, wB, h->wB, h->m3->m3, X,
+                       Y, Z);
+    HIPCHK(hipGetLastError());
+  } else {
+    HIPCHK(hipMemcpyAsync(h->wA, wA, h->n * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(h->wB, wB, h->n * 8, hipMemcpyDeviceToDevice, s));
+  }
+  if (h->sv->precision == CRYS_FP32) scft_prep<float>(h);
+  else scft_prep<double>(h);
+  HIPCHK(hipGetLastError());
+  h->fields_set = true;
+  return CRYS_OK;
+}
+
+// crys_scft_evaluate and crys_scft_evaluate_device: the fields and the field
+// gradients are host arrays, or (device) device arrays in the solver's
+// physical layout.
+int scft_evaluate_impl(crys_scft* h, bool device, const double* wA, const double* wB, double* H,
+                       double* dH_dwA, double* dH_dwB, double dH_dL[3], double* trace_row) {
+  if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
+  if ((wA == nullptr) != (wB == nullptr))
+    return fail(CRYS_ERR_INVALID_ARGUMENT, "pass both fields or neither");
+  if (wA) CHK(device ? scft_set_fields_device(h, wA, wB) : crys_scft_set_fields(h, wA, wB));
+  if (!h->fields_set) return fail(CRYS_ERR_INVALID_ARGUMENT, "no fields set");
+  crys_solver* sv = h->sv;
+  CHK(scft_ensure_ops(h));
+  OpSet& op = sv->ops[sv->cur];
+  hipStream_t s = sv->stream;
+  const size_t n = h->n, Ns = h->Ns;
+  const int nb = crys::red_blocks(n);
+  const bool stress = dH_dL != nullptr;
+  CHK(scft_storage(h));
+  if (!h->gA) HIPCHK(hipMalloc((void**)&h->gA, n * 8));
+  if (!h->gB) HIPCHK(hipMalloc((void**)&h->gB, n * 8));
+  if (!h->sums) HIPCHK(hipMalloc((void**)&h->sums, (1 + 3 * Ns) * 8));
+  if (stress) {
+    if (!h->spart) HIPCHK(hipMalloc((void**)&h->spart, 3 * Ns * nb * 8));
+    if (!op.dmat_dL[0]) CHK(upload_axis_ops(sv, op.ds, true, op.dmat_dL));
+  }
+  hipLaunchKernelGGL(crys::set_int_kernel, dim3(1), dim3(1), 0, s, h->counter, 0);
+  HIPCHK(hipGetLastError());
+  const bool f32 = sv->precision == CRYS_FP32;
+  CHK(f32 ? scft_record_iteration<float>(h, s, 0.0, stress)
+          : scft_record_iteration<double>(h, s, 0.0, stress));
+  const double cA = h->ds * (double)h->NsA / h->fA, cB = h->ds * (double)h->NsB / h->fB;
+  hipLaunchKernelGGL(crys::hgrad_kernel, dim3(nb), dim3(crys::kRedThreads), 0, s,
+                     (const double*)h->wA, (const double*)h->wB, (const double*)h->phiA,
+                     (const double*)h->phiB, h->gA, h->gB, h->qpart, n, h->xN, cA, cB);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(crys::sum_partials_kernel, dim3(1), dim3(crys::kRedThreads), 0, s,
+                     (const double*)h->qpart, nb, h->sums);
+  HIPCHK(hipGetLastError());
+  if (stress) {
+    hipLaunchKernelGGL(crys::sum_partials_kernel, dim3((unsigned)(3 * Ns)), dim3(crys::kRedThreads),
+                       0, s, (const double*)h->spart, nb, h->sums + 1);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<double> sums(1 + 3 * Ns);
+  double row[CRYS_TRACE_COLS];
+  HIPCHK(hipMemcpyAsync(row, h->trace, sizeof(row), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(sums.data(), h->sums, (stress ? 1 + 3 * Ns : 1) * 8,
+                        hipMemcpyDeviceToHost, s));
+  // a reduced value of a 3m solver with translational parts sets its 8
+  // images, which share one gradient: 8 times the full-grid gradient
+  if (device) {
+    double* out[2] = {dH_dwA, dH_dwB};
+    const double* g[2] = {h->gA, h->gB};
+    for (int k = 0; k < 2; ++k) {
+      if (!out[k]) continue;
+      if (h->m3) {
+        const int X = (int)h->m3->nphy[0], Y = (int)h->m3->nphy[1], Z = (int)h->m3->nphy[2];
+        hipLaunchKernelGGL(crys::m3_extract_kernel, dim3(grid_for(h->m3->n_phy)), dim3(256), 0, s,
+                           g[k], out[k], X, Y, Z, 8.0);
+        HIPCHK(hipGetLastError());
+      } else {
+        HIPCHK(hipMemcpyAsync(out[k], g[k], n * 8, hipMemcpyDeviceToDevice, s));
+      }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+  } else {
+    std::vector<double> gfa, gfb;
+    double* outA = dH_dwA;
+    double* outB = dH_dwB;
+    if (h->m3) {  // gradients on the full grid first
+      if (dH_dwA) {
+        gfa.resize(n);
+        outA = gfa.data();
+      }
+      if (dH_dwB) {
+        gfb.resize(n);
+        outB = gfb.data();
+      }
+    }
+    if (outA) HIPCHK(hipMemcpyAsync(outA, h->gA, n * 8, hipMemcpyDeviceToHost, s));
+    if (outB) HIPCHK(hipMemcpyAsync(outB, h->gB, n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (h->m3) {
+      if (dH_dwA) crys::m3_extract_host(gfa.data(), dH_dwA, h->m3->nphy, 8.0);
+      if (dH_dwB) crys::m3_extract_host(gfb.data(), dH_dwB, h->m3->nphy, 8.0);
+    }
+  }
+  const double Q = row[CRYS_TRACE_Q];
+  if (trace_row) memcpy(trace_row, row, sizeof(row));
+  if (H) *H = sums[0] / (double)n - log(Q);
+  if (stress) {
+    for (int a = 0; a < 3; ++a) {
+      double acc = 0.0;
+      for (size_t t = 0; t < Ns; ++t) acc += sums[1 + 3 * t + a];
+      dH_dL[a] = -acc / ((double)n * Q);
+    }
+  }
+  if (!std::isfinite(Q) || !(Q > 0)) return fail(CRYS_ERR_DIVERGED, "non-finite or zero Q");
+  return CRYS_OK;
 }
 
 }  // namespace
@@ -1120,6 +1272,7 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
   crys_scft* h = new (std::nothrow) crys_scft();
   if (!h) return fail(CRYS_ERR_OUT_OF_MEMORY, "host allocation failed");
   h->sv = solver;
+  h->user = solver;
   if (solver->full) {
     // 3m with translational parts: iterate the expanded fields with the P1
     // solver; every full-grid mean equals the reduced mean (each reduced point
@@ -1245,30 +1398,26 @@ int crys_scft_run(crys_scft_t* h, size_t nstep, double* trace_host) {
   if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
   if (!h->fields_set) return fail(CRYS_ERR_INVALID_ARGUMENT, "crys_scft_set_fields not called");
   crys_solver* sv = h->sv;
-  if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
-  const OpSet& op = sv->ops[sv->cur];
-  bool stale = !h->exec;
-  for (int a = 0; a < 3; ++a) stale = stale || h->graph_mats[a] != op.dmat[a];
-  if (stale) CHK(scft_build_graph(h));
+  CHK(scft_ensure_ops(h));
+  if (!h->exec || h->graph_gen != sv->ops_gen) CHK(scft_build_graph(h));
   hipStream_t s = sv->stream;
   bool diverged = false;
   size_t done = 0;
+  // without a trace buffer the rows still come back, for the divergence check
   std::vector<double> rows;
+  if (!trace_host) rows.resize((size_t)kTraceCap * CRYS_TRACE_COLS);
   while (done < nstep) {
     const size_t chunk = nstep - done < (size_t)kTraceCap ? nstep - done : (size_t)kTraceCap;
     hipLaunchKernelGGL(crys::set_int_kernel, dim3(1), dim3(1), 0, s, h->counter, 0);
     HIPCHK(hipGetLastError());
     for (size_t i = 0; i < chunk; ++i) HIPCHK(hipGraphLaunch(h->exec, s));
-    if (trace_host) {
-      HIPCHK(hipMemcpyAsync(trace_host + done * CRYS_TRACE_COLS, h->trace,
-                            chunk * CRYS_TRACE_COLS * 8, hipMemcpyDeviceToHost, s));
-    }
+    double* dst = trace_host ? trace_host + done * CRYS_TRACE_COLS : rows.data();
+    HIPCHK(hipMemcpyAsync(dst, h->trace, chunk * CRYS_TRACE_COLS * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (trace_host)
-      for (size_t i = 0; i < chunk; ++i) {
-        const double* r = trace_host + (done + i) * CRYS_TRACE_COLS;
-        if (!std::isfinite(r[CRYS_TRACE_F]) || !std::isfinite(r[CRYS_TRACE_Q])) diverged = true;
-      }
+    for (size_t i = 0; i < chunk; ++i) {
+      const double* r = dst + i * CRYS_TRACE_COLS;
+      if (!std::isfinite(r[CRYS_TRACE_F]) || !std::isfinite(r[CRYS_TRACE_Q])) diverged = true;
+    }
     done += chunk;
   }
   if (diverged) return fail(CRYS_ERR_DIVERGED, "non-finite free energy or Q in the trace");
@@ -1327,82 +1476,13 @@ int crys_diblock_scft_simple_iterate(crys_solver_t* solver, double fA, double ds
 
 int crys_scft_evaluate(crys_scft_t* h, const double* wA, const double* wB, double* H,
                        double* dH_dwA, double* dH_dwB, double dH_dL[3], double* trace_row) {
-  if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
-  if ((wA == nullptr) != (wB == nullptr))
-    return fail(CRYS_ERR_INVALID_ARGUMENT, "pass both fields or neither");
-  if (wA) CHK(crys_scft_set_fields(h, wA, wB));
-  if (!h->fields_set) return fail(CRYS_ERR_INVALID_ARGUMENT, "no fields set");
-  crys_solver* sv = h->sv;
-  if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
-  OpSet& op = sv->ops[sv->cur];
-  hipStream_t s = sv->stream;
-  const size_t n = h->n, Ns = h->Ns;
-  const int nb = crys::red_blocks(n);
-  const bool stress = dH_dL != nullptr;
-  CHK(scft_storage(h));
-  if (!h->gA) HIPCHK(hipMalloc((void**)&h->gA, n * 8));
-  if (!h->gB) HIPCHK(hipMalloc((void**)&h->gB, n * 8));
-  if (!h->sums) HIPCHK(hipMalloc((void**)&h->sums, (1 + 3 * Ns) * 8));
-  if (stress) {
-    if (!h->spart) HIPCHK(hipMalloc((void**)&h->spart, 3 * Ns * nb * 8));
-    if (!op.dmat_dL[0]) CHK(upload_axis_ops(sv, op.ds, true, op.dmat_dL));
-  }
-  hipLaunchKernelGGL(crys::set_int_kernel, dim3(1), dim3(1), 0, s, h->counter, 0);
-  HIPCHK(hipGetLastError());
-  const bool f32 = sv->precision == CRYS_FP32;
-  CHK(f32 ? scft_record_iteration<float>(h, s, 0.0, stress)
-          : scft_record_iteration<double>(h, s, 0.0, stress));
-  const double cA = h->ds * (double)h->NsA / h->fA, cB = h->ds * (double)h->NsB / h->fB;
-  hipLaunchKernelGGL(crys::hgrad_kernel, dim3(nb), dim3(crys::kRedThreads), 0, s,
-                     (const double*)h->wA, (const double*)h->wB, (const double*)h->phiA,
-                     (const double*)h->phiB, h->gA, h->gB, h->qpart, n, h->xN, cA, cB);
-  HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(crys::sum_partials_kernel, dim3(1), dim3(crys::kRedThreads), 0, s,
-                     (const double*)h->qpart, nb, h->sums);
-  HIPCHK(hipGetLastError());
-  if (stress) {
-    hipLaunchKernelGGL(crys::sum_partials_kernel, dim3((unsigned)(3 * Ns)), dim3(crys::kRedThreads),
-                       0, s, (const double*)h->spart, nb, h->sums + 1);
-    HIPCHK(hipGetLastError());
-  }
-  std::vector<double> sums(1 + 3 * Ns);
-  double row[CRYS_TRACE_COLS];
-  HIPCHK(hipMemcpyAsync(row, h->trace, sizeof(row), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(sums.data(), h->sums, (stress ? 1 + 3 * Ns : 1) * 8,
-                        hipMemcpyDeviceToHost, s));
-  std::vector<double> gfa, gfb;
-  double* outA = dH_dwA;
-  double* outB = dH_dwB;
-  if (h->m3) {  // gradients on the full grid first
-    if (dH_dwA) {
-      gfa.resize(n);
-      outA = gfa.data();
-    }
-    if (dH_dwB) {
-      gfb.resize(n);
-      outB = gfb.data();
-    }
-  }
-  if (outA) HIPCHK(hipMemcpyAsync(outA, h->gA, n * 8, hipMemcpyDeviceToHost, s));
-  if (outB) HIPCHK(hipMemcpyAsync(outB, h->gB, n * 8, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  if (h->m3) {
-    // a reduced value sets its 8 images, which share one gradient
-    if (dH_dwA) crys::m3_extract_host(gfa.data(), dH_dwA, h->m3->nphy, 8.0);
-    if (dH_dwB) crys::m3_extract_host(gfb.data(), dH_dwB, h->m3->nphy, 8.0);
-  }
-  const double Q = row[CRYS_TRACE_Q];
-  if (trace_row) memcpy(trace_row, row, sizeof(row));
-  if (H) *H = sums[0] / (double)n - log(Q);
-  if (stress) {
-    for (int a = 0; a < 3; ++a) {
-      double acc = 0.0;
-      for (size_t t = 0; t < Ns; ++t) acc += sums[1 + 3 * t + a];
-      dH_dL[a] = -acc / ((double)n * Q);
-    }
-  }
-  if (!std::isfinite(Q) || !(Q > 0)) return fail(CRYS_ERR_DIVERGED, "non-finite or zero Q");
-  return CRYS_OK;
+  return scft_evaluate_impl(h, false, wA, wB, H, dH_dwA, dH_dwB, dH_dL, trace_row);
+}
+
+int crys_scft_evaluate_device(crys_scft_t* h, const double* wA, const double* wB, double* H,
+                              double* dH_dwA, double* dH_dwB, double dH_dL[3],
+                              double* trace_row) {
+  return scft_evaluate_impl(h, true, wA, wB, H, dH_dwA, dH_dwB, dH_dL, trace_row);
 }
 
 int crys_axis_operator(int kind, size_t n_logical, double L, double ds, double* out) {
@@ -1426,7 +1506,7 @@ int crys_scft_time_propagation(crys_scft_t* h, int reps, double* ms_per_step,
                                 int* launches_per_step) {
   if (!h || reps <= 0) return fail(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
   crys_solver* sv = h->sv;
-  if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
+  CHK(scft_ensure_ops(h));
   CHK(scft_storage(h));
   hipStream_t s = sv->stream;
   // the propagation of both chains exactly as the iteration records it (one
@@ -1471,7 +1551,7 @@ int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* m
                         int* nkernels) {
   if (!h || reps <= 0) return fail(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
   crys_solver* sv = h->sv;
-  if (sv->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "solver lost its contour step");
+  CHK(scft_ensure_ops(h));
   const OpSet& op = sv->ops[sv->cur];
   hipStream_t s = sv->stream;
   CHK(scft_storage(h));

@@ -97,15 +97,15 @@ static __global__ void m3_expand_kernel(const double* __restrict__ red, double* 
   }
 }
 
-// red[r] = full[2 r]
+// red[r] = scale * full[2 r]
 static __global__ void m3_extract_kernel(const double* __restrict__ full, double* __restrict__ red,
-                                         int X, int Y, int Z) {
+                                         int X, int Y, int Z, double scale) {
   const size_t n = (size_t)X * Y * Z;
   const size_t NY = 2 * (size_t)Y, NZ = 2 * (size_t)Z;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const size_t x = i / ((size_t)Y * Z), y = (i / Z) % Y, z = i % Z;
-    red[i] = full[((2 * x) * NY + 2 * y) * NZ + 2 * z];
+    red[i] = scale * full[((2 * x) * NY + 2 * y) * NZ + 2 * z];
   }
 }
 

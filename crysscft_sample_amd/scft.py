@@ -121,6 +121,18 @@ class SCFT:
                                        dptr(gB), None if dL is None else dptr(dL), dptr(row)))
         return {"H": H.value, "dH_dwA": gA, "dH_dwB": gB, "dH_dL": dL, "trace_row": row}
 
+    def evaluate_device(self, wA_ptr, wB_ptr, gA_ptr, gB_ptr, stress=True):
+        """crys_scft_evaluate_device: fields and gradients are device addresses
+        (ints) of space_size_phy doubles; the inputs must be ready.  Returns a
+        dict with ``H``, ``dH_dL`` (3,) or None and ``trace_row``."""
+        H = ctypes.c_double()
+        dL = np.empty(3) if stress else None
+        row = np.empty(_lib.TRACE_COLS)
+        check(lib().crys_scft_evaluate_device(self._h, wA_ptr, wB_ptr, ctypes.byref(H), gA_ptr,
+                                              gB_ptr, None if dL is None else dptr(dL),
+                                              dptr(row)))
+        return {"H": H.value, "dH_dL": dL, "trace_row": row}
+
     def time_propagation(self, reps: int = 5):
         """(ms per contour step of both chains, launches per step) over the
         iteration's own launch sequence, captured in a hipGraph."""

@@ -42,6 +42,8 @@ class CrysPSSolverBase:
                                        ctypes.byref(h)))
         self._h = h
         self.precision = prec
+        # PmmmDCT keeps only cell_para[0] unless orthorhombic (solver_Pmmm.h:101)
+        self.cubic_only = self.kind == _lib.PMMM_DCT and not orthorhombic
 
     # -- lifetime --------------------------------------------------------
     def close(self):
@@ -117,6 +119,12 @@ class CrysPSSolverBase:
         v = ctypes.c_double()
         check(lib().crys_get_ds(self.handle, ctypes.byref(v)))
         return float(v.value)
+
+    def effective_cell(self, cell_para):
+        """The cell lengths the solver applies for `cell_para`: (L0, L0, L0)
+        for a PmmmDCT solver that keeps the reference's cubic quirk."""
+        c = tuple(float(v) for v in cell_para[:3])
+        return (c[0],) * 3 if self.cubic_only else c
 
     def get_iomat_for_diffusion(self) -> int:
         """Device address of the solver-owned in/out buffer."""

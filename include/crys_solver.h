@@ -297,6 +297,16 @@ CRYS_API int crys_expand_fields(int kind, const size_t grid_log[3], const double
 CRYS_API int crys_scft_evaluate(crys_scft_t* h, const double* wA, const double* wB, double* H,
                                 double* dH_dwA, double* dH_dwB, double dH_dL[3],
                                 double* trace_row);
+/* crys_scft_evaluate with the fields wA, wB and the gradients dH_dwA, dH_dwB
+ * in DEVICE memory (space_size_phy doubles each, the solver's physical
+ * layout), for callers whose tensors already live on the GPU (the torch
+ * binding); H, dH_dL and trace_row stay host values.  The inputs must be
+ * ready when the call is made (work that writes them on another stream is
+ * finished); the call returns after the outputs are written.  Same results
+ * as crys_scft_evaluate. */
+CRYS_API int crys_scft_evaluate_device(crys_scft_t* h, const double* wA_dev,
+                                       const double* wB_dev, double* H, double* dH_dwA_dev,
+                                       double* dH_dwB_dev, double dH_dL[3], double* trace_row);
 
 /* ---- introspection (host only, no device needed) ------------------------ */
 

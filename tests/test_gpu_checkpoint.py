@@ -197,6 +197,45 @@ def test_evaluate_every_12_slots():
     s.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_differentiation_workload_gradient_against_oracle(oracle, prec):
+    """The differentiation workload at its full size (64^3, orthorhombic,
+    Ns 35 + 92, checkpoints every 12 slots): dH/dw against the oracle's own
+    densities, d(-ln Q)/dwA = (ds NsA / fA) phiA / n (the oracle's P1 on the
+    full mirror-symmetric fields, which honours the orthorhombic cell), and
+    against central differences of H at a few points."""
+    wAf, wBf = F.bcc_fields(DN, DCELL, DFA, DXN)
+    a, b = F.extract(wAf, "PmmmDCT"), F.extract(wBf, "PmmmDCT")
+    s = C.SolverPmmmDCT(DN, orthorhombic=True, precision=prec)
+    run = C.SCFT(s, DFA, DDS, DXN, DCELL, 0.0)
+    run.set_checkpointing(12)
+    g = run.evaluate(a, b, stress=False)
+    tr, _, _, phiA, phiB = oracle.Solver("P1", DN).scft(DFA, DDS, DXN, DCELL, 0.0, 1, wAf, wBf)
+    phiA, phiB = F.extract(phiA, "PmmmDCT"), F.extract(phiB, "PmmmDCT")
+    n = a.size
+    cA, cB = DDS * 35 / DFA, DDS * 92 / (1 - DFA)
+    gA = ((a - b) / (2 * DXN) - 0.5 + cA * phiA) / n
+    gB = (-(a - b) / (2 * DXN) - 0.5 + cB * phiB) / n
+    H = float(np.mean((wAf - wBf) ** 2 / (4 * DXN) - (wAf + wBf) / 2 + DXN / 4) - np.log(tr[0, 6]))
+    eA, eB = rel_err(g["dH_dwA"], gA), rel_err(g["dH_dwB"], gB)
+    say(f"  {prec} 64^3 every 12: dH/dwA {eA:.2e}, dH/dwB {eB:.2e}, H {abs(g['H'] / H - 1):.2e}")
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    assert eA < tol and eB < tol
+    assert abs(g["H"] / H - 1) < (1e-11 if prec == "fp64" else 1e-6)
+    if prec == "fp64":
+        h = 2e-3
+        for idx in [(0, 0, 0), (11, 23, 5), (31, 2, 17)]:
+            def Hd(d):
+                w = a.copy()
+                w[idx] += d
+                return run.evaluate(w, b, stress=False)["H"]
+            fd = central4(Hd, h)
+            say(f"    dH/dwA{idx}: fd {fd:.10e} adjoint {g['dH_dwA'][idx]:.10e}")
+            assert abs(fd / g["dH_dwA"][idx] - 1) < 1e-6
+    run.close()
+    s.close()
+
+
 def test_glide_3m_checkpointed_equals_full():
     """3m with translational parts (run as P1 on the expanded fields) under
     checkpointing: iteration and gradient equal the full-history run."""

@@ -1,0 +1,179 @@
+"""SCFT handle life cycle on the GPU (needs an MI355X).
+
+  - the captured iteration graph follows the solver's operators: a run, a new
+    cell (box relaxation), and a run again equals a fresh handle at the new
+    cell, also when the solver's operator addresses are reused;
+  - a handle keeps its own contour step when the caller gives the solver
+    another one (E, NsA and NsB belong to the handle's ds);
+  - a non-finite run is reported without a trace buffer too;
+  - the one-launch phases on grids whose axes differ inside one length class
+    (row axis length != z length, x-slab and y-slab phases with different
+    plane counts), single handle and batch, against the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+import crysscft_sample_amd as C
+from crysscft_sample_amd import _lib
+from crysscft_sample_amd import fields as F
+from crysscft_sample_amd.batch import Batch, Instance
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def seeded(name, N, cell, fA, xN, seed=0):
+    wA, wB = F.bcc_fields(N, cell, fA, xN, seed=seed)
+    return F.extract(wA, name), F.extract(wB, name)
+
+
+@pytest.mark.parametrize("name,N", [("P1", (32, 32, 32)), ("PmmmDCT", (64, 64, 64)),
+                                    ("PmmmDCT", (128, 128, 128))])
+@pytest.mark.parametrize("caller_sets_ds", [True, False])
+def test_new_cell_rebuilds_the_iteration(name, N, caller_sets_ds):
+    """run, set_cell_para, (set_contour_step), run == a fresh handle at the new
+    cell from the fields the first run left (class 32 and 64 centrosymmetric
+    grids: the graph reads the half-size operators, not the dense ones)."""
+    fA, xN, ds, acc = 0.3, 25.0, 0.01, 0.1
+    c1, c2 = (4.2,) * 3, (4.35,) * 3
+    a, b = seeded(name, N, c1, fA, xN)
+    s = C.SOLVERS[name](N)
+    run = C.SCFT(s, fA, ds, xN, c1, acc)
+    run.set_fields(a, b)
+    run.run(2)
+    _, _, wA1, wB1 = run.fields()
+    for _ in range(2):  # twice: the allocator may hand back the freed addresses
+        s.set_cell_para(c2)
+        if caller_sets_ds:
+            s.set_contour_step(ds)
+        s.set_cell_para(c1)
+        s.set_contour_step(ds)
+        s.set_cell_para(c2)
+        if caller_sets_ds:
+            s.set_contour_step(ds)
+    got = run.run(2)
+    assert s.get_ds() == ds
+    s2 = C.SOLVERS[name](N)
+    fresh = C.SCFT(s2, fA, ds, xN, c2, acc)
+    fresh.set_fields(wA1, wB1)
+    want = fresh.run(2)
+    say(f"  {name} {N}: after the cell change {rel_err(got, want):.2e}")
+    np.testing.assert_array_equal(got, want)
+
+
+def test_handle_keeps_its_contour_step():
+    """A second handle with another ds sets the shared solver to that ds; the
+    first handle still runs with its own (the oracle of a fresh run)."""
+    N, cell, fA, xN, acc = (32, 32, 32), (4.0,) * 3, 0.3, 25.0, 0.1
+    a, b = seeded("PmmmDCT", N, cell, fA, xN)
+    s = C.SolverPmmmDCT(N)
+    h1 = C.SCFT(s, fA, 0.01, xN, cell, acc)
+    h1.set_fields(a, b)
+    h2 = C.SCFT(s, fA, 0.02, xN, cell, acc)
+    h2.set_fields(a, b)
+    assert s.get_ds() == 0.02
+    t1 = h1.run(2)
+    assert s.get_ds() == 0.01
+    t2 = h2.run(2)
+    assert s.get_ds() == 0.02
+    t1b = h1.evaluate(a, b, stress=False)["trace_row"]
+    ref1 = C.SCFT(C.SolverPmmmDCT(N), fA, 0.01, xN, cell, acc)
+    ref1.set_fields(a, b)
+    r1 = ref1.run(2)
+    ref2 = C.SCFT(C.SolverPmmmDCT(N), fA, 0.02, xN, cell, acc)
+    ref2.set_fields(a, b)
+    r2 = ref2.run(2)
+    np.testing.assert_array_equal(t1, r1)
+    np.testing.assert_array_equal(t2, r2)
+    np.testing.assert_array_equal(t1b, r1[0])
+
+
+def test_divergence_without_trace():
+    N = (32, 32, 32)
+    a, b = seeded("PmmmDCT", N, (4.0,) * 3, 0.3, 20.0)
+    s = C.SolverPmmmDCT(N)
+    run = C.SCFT(s, 0.3, 0.01, 20.0, (4.0,) * 3, 0.1)
+    bad = a.copy()
+    bad[1, 2, 3] = np.inf
+    run.set_fields(bad, b)
+    with pytest.raises(C.CrysError) as e:
+        run.run(3, trace=False)
+    assert e.value.status == _lib.ERR_DIVERGED
+    run.set_fields(a, b)
+    assert run.run(2, trace=False) is None
+
+
+# Axes that differ inside one length class (48: lengths 33..48; 32: 17..32)
+NONCUBIC = [("PmmmDCT", (96, 80, 96), (4.2,) * 3),     # reduced 48 x 40 x 48
+            ("P1", (40, 48, 48), (4.0, 4.4, 4.3)),     # x 40, y and z 48
+            ("3m", (80, 96, 96), (4.1, 4.4, 4.3)),     # reduced 40 x 48 x 48
+            ("P1", (48, 40, 48), (4.3, 4.0, 4.4)),     # y 40: row axis of x-slabs != z
+            ("PmmmDCT", (64, 48, 64), (4.0,) * 3)]     # reduced 32 x 24 x 32
+NC_PARAMS = dict(fA=0.3, xN=25.0, ds=0.02, acc=0.1, nstep=2)
+_NC_REF = {}
+
+
+def noncubic_ref(oracle, k):
+    """(fields, oracle trace, oracle phiA) of NONCUBIC[k], computed once."""
+    if k not in _NC_REF:
+        name, N, cell = NONCUBIC[k]
+        p = NC_PARAMS
+        a, b = seeded(name, N, cell, p["fA"], p["xN"], seed=40 + k)
+        tr, _, _, phiA, _ = oracle.Solver(name, N).scft(p["fA"], p["ds"], p["xN"], cell, p["acc"],
+                                                        p["nstep"], a, b)
+        _NC_REF[k] = (a, b, tr, phiA)
+    return _NC_REF[k]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("k", range(len(NONCUBIC)))
+def test_phases_on_non_cubic_grids(oracle, k, prec):
+    name, N, cell = NONCUBIC[k]
+    p = NC_PARAMS
+    a, b, tr, phiA = noncubic_ref(oracle, k)
+    s = C.SOLVERS[name](N, precision=prec)
+    run = C.SCFT(s, p["fA"], p["ds"], p["xN"], cell, p["acc"])
+    run.set_fields(a, b)
+    got = run.run(p["nstep"])
+    _, launches = run.time_propagation(reps=1)
+    gphi = run.fields()[0]
+    dF = np.abs(got[:, 0] / tr[:, 0] - 1).max()
+    dM = np.abs(got[:, 5] / tr[:, 5] - 1).max()
+    say(f"  {name} {N} {prec}: {launches} launch(es) per step, |dF/F| {dF:.2e} "
+        f"|dMaxInc| {dM:.2e} |dphiA| {np.abs(gphi - phiA).max():.2e}")
+    assert launches == 1
+    if prec == "fp64":
+        assert dF <= 1e-10 and dM <= 1e-10
+        assert rel_err(gphi, phiA) < 1e-10
+    else:
+        assert dF < 1e-6
+        assert np.abs(gphi - phiA).max() < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_phase_batch_on_non_cubic_grids(oracle, prec):
+    p = NC_PARAMS
+    insts = [Instance(name, N, cell, p["fA"], p["xN"], p["ds"], p["acc"])
+             for name, N, cell in NONCUBIC]
+    b = Batch(insts, precision=prec)
+    assert b.path() == "phases"
+    refs = [noncubic_ref(oracle, k) for k in range(len(insts))]
+    for k, r in enumerate(refs):
+        b.set_fields(k, r[0], r[1])
+    tr = b.run(p["nstep"])
+    for k, inst in enumerate(insts):
+        t_ref = refs[k][2]
+        dF = np.abs(tr[k, :, 0] / t_ref[:, 0] - 1).max()
+        dM = np.abs(tr[k, :, 5] / t_ref[:, 5] - 1).max()
+        say(f"  batch {prec} {inst.solver} {inst.grid_log}: |dF/F| {dF:.2e} |dMaxInc| {dM:.2e}")
+        if prec == "fp64":
+            assert dF <= 1e-10 and dM <= 1e-10
+        else:
+            assert dF < 1e-6
+    b.close()
