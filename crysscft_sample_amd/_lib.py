@@ -76,6 +76,7 @@ SIGNATURES = {
     "crys_solver_synchronize": (ctypes.c_int, [_c_vp]),
     "crys_solver_get_stream": (ctypes.c_int, [_c_vp, ctypes.POINTER(_c_vp)]),
     "crys_propagate": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_sz, ctypes.c_int, _c_vp]),
+    "crys_propagate_noalloc": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_sz, ctypes.c_int, _c_vp]),
     "crys_scft_create": (ctypes.c_int, [_c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.POINTER(_c_vp)]),

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -140,6 +141,7 @@ struct crys_solver {
   double ds_current = 0.0;              // solver_base.h:46
   double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   hipStream_t stream = nullptr;
+  int device = 0;           // the HIP device current at creation
   double* iomat = nullptr;  // device, n_phy doubles
   void* work = nullptr;     // device, n_phy elements of the precision
   void* scratch = nullptr;  // device, n_phy elements of the precision
@@ -542,6 +544,46 @@ int propagate_impl(crys_solver_t* h, const double* E, const double* init, size_t
 
 }  // namespace
 
+// Staging field of crys_propagate_noalloc: the half-finished contour step
+// between the axis passes, one field of the largest physical grid a solver
+// accepts (every axis at most 128 points: 2^21 doubles, 16 MiB).  Each GPU
+// the library runs on holds its own copy until the process ends; every call
+// on one GPU shares it.  Host code uses its device address only (per device,
+// from hipGetSymbolAddress), never the name.
+constexpr size_t kNoAllocMaxPoints = (size_t)1 << 21;
+static __device__ __align__(256) double scratch[kNoAllocMaxPoints];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev`, looked up on the first
+// call for that device with that device current, then kept.
+int noalloc_scratch(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(CRYS_ERR_UNSUPPORTED, "device %d is beyond the %d the library tracks", dev,
+                kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(CRYS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return CRYS_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* crys_last_error(void) { return g_err.c_str(); }
@@ -621,7 +663,8 @@ int crys_solver_create(int kind, const size_t grid_log[3], const double translat
                            "onto the grid");
     }
   }
-  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&h->iomat, h->n_phy * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->work, h->n_phy * elem_size(precision));
   if (e == hipSuccess) e = hipMalloc(&h->scratch, h->n_phy * elem_size(precision));
@@ -837,6 +880,18 @@ int crys_propagate(crys_solver_t* h, const double* E, const double* init, size_t
                    int direction, double* q) {
   CHK(check_handle(h));
   return propagate_impl(h, E, init, Ns, direction, q, (double*)h->scratch);
+}
+
+int crys_propagate_noalloc(crys_solver_t* h, const double* E, const double* init, size_t Ns,
+                           int direction, double* q) {
+  CHK(check_handle(h));
+  if (h->n_phy > kNoAllocMaxPoints)
+    return fail(CRYS_ERR_UNSUPPORTED,
+                "crys_propagate_noalloc: %zu physical points exceed its static staging field of "
+                "%zu points", h->n_phy, kNoAllocMaxPoints);
+  double* scr = nullptr;
+  CHK(noalloc_scratch(h->device, &scr));
+  return propagate_impl(h, E, init, Ns, direction, q, scr);
 }
 
 }  // extern "C"

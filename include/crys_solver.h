@@ -138,6 +138,19 @@ CRYS_API int crys_solver_get_stream(const crys_solver_t* h, void** out_stream);
 CRYS_API int crys_propagate(crys_solver_t* h, const double* E_dev, const double* init_dev, size_t Ns,
                    int direction, double* q_hist_dev);
 
+/* crys_propagate without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and results as crys_propagate, but the half-finished contour step is
+ * staged in the library's static device field of 2^21 doubles (16 MiB per
+ * GPU, held until the process ends) instead of the solver's own scratch field;
+ * solvers with more physical points (none that crys_solver_create accepts
+ * today) get CRYS_ERR_UNSUPPORTED.  The call makes no hipMalloc or hipFree.
+ * All calls on one GPU share that field: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller. */
+CRYS_API int crys_propagate_noalloc(crys_solver_t* h, const double* E_dev, const double* init_dev,
+                                    size_t Ns, int direction, double* q_hist_dev);
+
 /* ---- SCFT iteration (diblock_scft_simple_iterate) ---------------------- */
 
 /* Columns of one trace row, one row per iteration (print_energy,
