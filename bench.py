@@ -289,6 +289,7 @@ def batch_timing(count, steps):
         out[prec] = {"instance_iter_s": round(count * steps / dt, 2),
                      "ms_per_batch_iteration": round(dt / steps * 1e3, 3),
                      "path": path,
+                     "phase_grid": b.phase_grid(),
                      "autotune_ms_steps": {"phases": None if ms_ph is None else round(ms_ph, 3),
                                            "passes": round(ms_pa, 3)}}
         b.close()
@@ -299,6 +300,35 @@ def batch_timing(count, steps):
     dphi = np.array([np.abs(a - b).max() for a, b in zip(phis["fp32"], phis["fp64"])])
     out["fp32_vs_fp64"] = {"max_rel_dF": float(dF.max()), "median_rel_dF": float(np.median(dF)),
                            "max_abs_dphiA": float(dphi.max())}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+SYNC_VARIANTS = ["launch", "empty_launch", "fenced", "two_level", "bypass", "barrier_only"]
+
+
+def sync_timing(rounds):
+    """One JSON line: microseconds per round of the exchange between contour
+    steps (crys_time_grid_sync) for each synchronisation variant at 64, 128
+    and 256 workgroups of 1024 threads."""
+    import ctypes
+
+    from crysscft_sample_amd._lib import check, lib
+
+    out = {"metric": "us per round: 64 KB per workgroup written, then read by a workgroup on "
+                     "another XCD, synchronised by a launch or a grid barrier",
+           "unit": "us", "higher_is_better": False, "rounds": rounds, "workgroups": {}}
+    for wg in (64, 128, 256):
+        row = {}
+        for v, name in enumerate(SYNC_VARIANTS):
+            us = ctypes.c_double()
+            bad, tmo = ctypes.c_uint(), ctypes.c_uint()
+            print(f"bench: grid sync {name} {wg} workgroups", file=sys.stderr, flush=True)
+            check(lib().crys_time_grid_sync(v, wg, rounds, ctypes.byref(us), ctypes.byref(bad),
+                                            ctypes.byref(tmo)))
+            row[name] = {"us": round(us.value, 3), "not_visible": bad.value,
+                         "timeouts": tmo.value}
+        out["workgroups"][str(wg)] = row
     print(json.dumps(out), flush=True)
     return 0
 
@@ -398,6 +428,9 @@ def main():
                     help="with --passes: time these grids instead (e.g. PmmmDCT:256, 3m:64:4.0)")
     ap.add_argument("--pass-path", action="append", choices=list(PASS_PATHS),
                     help="with --passes: time only these kernel paths")
+    ap.add_argument("--sync", action="store_true",
+                    help="time the exchange between contour steps under a launch and under "
+                         "grid barriers (crys_time_grid_sync) and exit")
     ap.add_argument("--batch", type=int, default=0, metavar="COUNT",
                     help="time COUNT batched instances (the batching workload) and exit")
     args = ap.parse_args()
@@ -405,6 +438,8 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not run "
               "(the path would shard by independent SCFT instances, one batch per GPU)", file=sys.stderr)
         return 2
+    if args.sync:
+        return sync_timing(400)
     if args.batch:
         return batch_timing(args.batch, max(args.steps // 10, 3))
     if args.pmc_workload:

@@ -108,12 +108,17 @@ SIGNATURES = {
     "crys_batch_set_path": (ctypes.c_int, [_c_vp, ctypes.c_int]),
     "crys_batch_get_path": (ctypes.c_int, [_c_vp, _c_ip]),
     "crys_batch_autotune": (ctypes.c_int, [_c_vp, ctypes.c_int, _c_ip, _c_dp]),
+    "crys_batch_set_phase_grid": (ctypes.c_int, [_c_vp, ctypes.c_int]),
+    "crys_batch_get_phase_grid": (ctypes.c_int, [_c_vp, _c_ip]),
     "crys_batch_run": (ctypes.c_int, [_c_vp, _c_sz, _c_dp]),
     "crys_batch_get_fields": (ctypes.c_int, [_c_vp, _c_sz, _c_dp, _c_dp, _c_dp, _c_dp]),
     "crys_batch_get_info": (ctypes.c_int, [_c_vp, _c_sz, ctypes.POINTER(_c_sz),
                                            ctypes.POINTER(_c_sz), ctypes.POINTER(_c_sz)]),
     "crys_scft_evaluate": (ctypes.c_int, [_c_vp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp,
                                           _c_dp]),
+    "crys_time_grid_sync": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dp,
+                                           ctypes.POINTER(ctypes.c_uint),
+                                           ctypes.POINTER(ctypes.c_uint)]),
     "crys_scft_evaluate_device": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_dp, _c_vp, _c_vp, _c_dp,
                                                  _c_dp]),
 }

@@ -109,6 +109,16 @@ class Batch:
         check(lib().crys_batch_autotune(self._h, int(reps), ctypes.byref(p), ms))
         return ({1: "phases", 2: "passes"}[p.value], ms[0] if ms[0] >= 0 else None, ms[1])
 
+    def set_phase_grid(self, workgroups: int):
+        """0: one workgroup per phase item; G > 0: a persistent grid of G
+        workgroups that prefetch the next item's plane into LDS."""
+        check(lib().crys_batch_set_phase_grid(self._h, int(workgroups)))
+
+    def phase_grid(self) -> int:
+        g = ctypes.c_int()
+        check(lib().crys_batch_get_phase_grid(self._h, ctypes.byref(g)))
+        return g.value
+
     def fields(self, k):
         sh = self.shape(k)
         phiA, phiB, wA, wB = (np.empty(sh) for _ in range(4))

@@ -17,7 +17,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB_NAME = "libcrys_solver.so"
 LIB_PATH = os.path.join(PKG, LIB_NAME)
 
-HIP_SOURCES = ["crys_solver.hip", "batch.hip"]
+HIP_SOURCES = ["crys_solver.hip", "batch.hip", "grid_sync.hip"]
 CXX_SOURCES = ["operators.cpp", "field_io.cpp"]
 
 ARCH = os.environ.get("CRYS_OFFLOAD_ARCH", "gfx950")

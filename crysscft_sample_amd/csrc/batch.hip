@@ -51,6 +51,16 @@ size_t phase_batch_items() {
   return lim;
 }
 
+// CRYS_BATCH_PERSISTENT=0 runs the batch phases with one workgroup per item
+// instead of the persistent grid (for A/B timing).
+bool batch_persistent_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("CRYS_BATCH_PERSISTENT");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
 int errf(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -197,6 +207,149 @@ __global__ __launch_bounds__(NT) void batch_phase_kernel(const BatchInst<T>* __r
       if constexpr (NT == 512)
         plane_phase_body<T, 64, false, NT>(ch, mr, mc, lr, lc, MR, MC, base, RS, lds);
       break;
+  }
+}
+
+// Copies one plane (rows of MC elements at stride RS from in + base) into
+// `raw` as compact rows (element (r, c) at r * MC + c) with gfx950's direct
+// global-to-LDS loads: 16 bytes per lane, each wave's load one contiguous
+// 1 KB block of LDS, no registers.  Lanes past the plane's end reload its
+// first element into LDS slots past the plane (raw holds kRawElems, a whole
+// number of blocks).  The copy is complete once every wave has waited for its
+// vector memory counter and the workgroup has passed a barrier.
+constexpr int kRawElems = 64 * 64;
+
+template <typename T, int NT>
+__device__ __forceinline__ void dma_plane(const T* in, long base, long RS, int MR, int MC,
+                                          T* raw) {
+  constexpr int PER = 16 / (int)sizeof(T);
+  constexpr int BLK = 64 * PER;
+  static_assert(kRawElems % BLK == 0, "raw holds whole blocks");
+  const int total = MR * MC;
+  const int blocks = (total + BLK - 1) / BLK;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int j = w; j < blocks; j += NT / 64) {
+    int e = j * BLK + lane * PER;
+    if (e >= total) e = 0;
+    const int r = e / MC, c = e - r * MC;
+    __builtin_amdgcn_global_load_lds(in + base + r * RS + c, raw + j * BLK, 16, 0, 0);
+  }
+}
+
+// Geometry of phase item `item` (instance, chain << 16 | plane) of phase k.
+template <typename T>
+struct PhaseItem {
+  const BatchInst<T>* I;
+  PhaseChain<T> ch;
+  int ra, MR, MC;
+  long base, RS;
+};
+
+template <typename T>
+__device__ __forceinline__ PhaseItem<T> phase_item(const BatchInst<T>* __restrict__ insts,
+                                                   const int2* __restrict__ work, int item, int k,
+                                                   int yslab) {
+  PhaseItem<T> it;
+  const int2 w = work[item];
+  const BatchInst<T>& I = insts[w.x];
+  it.I = &I;
+  const int chain = w.y >> 16, plane = w.y & 0xffff;
+  const long n = I.n;
+  const int Ns = I.Ns, NsA = I.NsA;
+  T* q = chain == 0 ? I.qf : I.qb;
+  T* half[2] = {chain == 0 ? I.scr0 : I.scr1, chain == 0 ? I.scr0b : I.scr1b};
+  auto E = [&](int t) -> const T* {
+    return chain == 0 ? (t < NsA ? I.EA : I.EB) : (Ns - t - 1 >= NsA ? I.EB : I.EA);
+  };
+  it.ch = PhaseChain<T>{};
+  it.ch.in = k == 0 ? (chain == 0 ? q : q + (long)Ns * n) : half[(k + 1) % 2];
+  if (k >= 1) {
+    it.ch.hist = chain == 0 ? q + (long)k * n : q + (long)(Ns - k) * n;
+    it.ch.e_fin = E(k - 1);
+  }
+  if (k < Ns) {
+    it.ch.out = half[k % 2];
+    it.ch.e_next = E(k);
+  }
+  it.ra = yslab ? 0 : 1;
+  it.MR = yslab ? I.X : I.Y;
+  it.MC = I.Z;
+  it.base = yslab ? (long)plane * I.Z : (long)plane * I.Y * I.Z;
+  it.RS = yslab ? (long)I.Y * I.Z : (long)I.Z;
+  return it;
+}
+
+// Phase k of the batch on a persistent grid: workgroup g takes the items
+// g, g + G, g + 2 G, ... of the phase's work list (G = gridDim.x, a few per
+// CU).  Each item's plane is copied into LDS (dma_plane) while the previous
+// item is being computed, so that the plane's load latency, exposed once per
+// item when every item is a workgroup of its own, is hidden behind the
+// products.  The items of one phase are independent, so no workgroup waits
+// for another.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void batch_phase_persistent_kernel(
+    const BatchInst<T>* __restrict__ insts, const int2* __restrict__ work, int cnt, int k,
+    int yslab) {
+  __shared__ T lds[64 * PhaseShape<T, 64, false, NT>::PB];
+  __shared__ __align__(16) T raw[kRawElems];
+  int item = blockIdx.x;
+  if (item >= cnt) return;
+  {
+    const PhaseItem<T> f = phase_item<T>(insts, work, item, k, yslab);
+    dma_plane<T, NT>(f.ch.in, f.base, f.RS, f.MR, f.MC, raw);
+  }
+  for (; item < cnt; item += gridDim.x) {
+    // the plane of this item has landed in raw (every wave's copy), and the
+    // previous item is done with lds
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const PhaseItem<T> it = phase_item<T>(insts, work, item, k, yslab);
+    const BatchInst<T>& I = *it.I;
+    const int nx = item + (int)gridDim.x;
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));  // opaque: recomputed per plane (see plane_phase_body)
+    auto prefetch = [&]() {
+      if (nx < cnt) {
+        const PhaseItem<T> nt = phase_item<T>(insts, work, nx, k, yslab);
+        dma_plane<T, NT>(nt.ch.in, nt.base, nt.RS, nt.MR, nt.MC, raw);
+      }
+    };
+    if (I.cen) {
+      const T *mr = I.mat_c[it.ra], *mc = I.mat_c[2];
+      const unsigned short *lr = I.mat_c_lo[it.ra], *lc = I.mat_c_lo[2];
+      const bool sh = I.shift != 0;
+      if (I.pcls == 32)
+        plane_phase_body<T, 32, true, NT, true>(it.ch, mr, mc, lr, lc, it.MR, it.MC, it.base,
+                                                it.RS, lds, sh, raw, prefetch, tid);
+      else
+        plane_phase_body<T, 64, true, NT, true>(it.ch, mr, mc, lr, lc, it.MR, it.MC, it.base,
+                                                it.RS, lds, sh, raw, prefetch, tid);
+      continue;
+    }
+    const T *mr = I.mat[it.ra], *mc = I.mat[2];
+    const unsigned short *lr = I.mat_lo[it.ra], *lc = I.mat_lo[2];
+    switch (I.pcls) {
+      case 16:
+        plane_phase_body<T, 16, false, NT, true>(it.ch, mr, mc, lr, lc, it.MR, it.MC, it.base,
+                                                 it.RS, lds, false, raw, prefetch, tid);
+        break;
+      case 32:
+        plane_phase_body<T, 32, false, NT, true>(it.ch, mr, mc, lr, lc, it.MR, it.MC, it.base,
+                                                 it.RS, lds, false, raw, prefetch, tid);
+        break;
+      case 48:
+        plane_phase_body<T, 48, false, NT, true>(it.ch, mr, mc, lr, lc, it.MR, it.MC, it.base,
+                                                 it.RS, lds, false, raw, prefetch, tid);
+        break;
+      default:
+        // the host runs the NT = 512 kernel whenever a dense class-64 plane exists
+        if constexpr (NT == 512)
+          plane_phase_body<T, 64, false, NT, true>(it.ch, mr, mc, lr, lc, it.MR, it.MC, it.base,
+                                                   it.RS, lds, false, raw, prefetch, tid);
+        else
+          prefetch();
+        break;
+    }
   }
 }
 
@@ -356,6 +509,10 @@ struct crys_batch {
   bool phase_ok = false;                          // every instance can run the phases
   bool phase_dense64 = false;                     // a dense class-64 plane: 512-thread phases
   bool phases = false;                            // one launch per step (batch_phase_kernel)
+  // phases on a persistent grid of this many workgroups
+  // (batch_phase_persistent_kernel), or 0: one workgroup per item
+  int phase_grid = 0;
+  int n_cu = 0;                                   // compute units of the batch's device
   int2* pwork[2] = {nullptr, nullptr};            // phase items: x-slab, y-slab phases
   std::vector<int> pprefix[2];                    // phase items active at phase k
   hipStream_t stream = nullptr;
@@ -667,12 +824,21 @@ int batch_record_steps(crys_batch* b, hipStream_t s, bool phases) {
     const int ys = k % 2;
     const int cnt = b->pprefix[ys][k];
     if (cnt == 0) continue;
-    if (b->phase_dense64)
+    if (b->phase_grid > 0) {
+      const unsigned g = (unsigned)std::min(cnt, b->phase_grid);
+      if (b->phase_dense64)
+        hipLaunchKernelGGL((crys::batch_phase_persistent_kernel<T, 512>), dim3(g), dim3(512), 0,
+                           s, I, (const int2*)b->pwork[ys], cnt, k, ys);
+      else
+        hipLaunchKernelGGL((crys::batch_phase_persistent_kernel<T, 1024>), dim3(g), dim3(1024),
+                           0, s, I, (const int2*)b->pwork[ys], cnt, k, ys);
+    } else if (b->phase_dense64) {
       hipLaunchKernelGGL((crys::batch_phase_kernel<T, 512>), dim3(cnt), dim3(512), 0, s, I,
                          (const int2*)b->pwork[ys], k, ys);
-    else
+    } else {
       hipLaunchKernelGGL((crys::batch_phase_kernel<T, 1024>), dim3(cnt), dim3(1024), 0, s, I,
                          (const int2*)b->pwork[ys], k, ys);
+    }
     BHIP(hipGetLastError());
   }
   for (int t = 0; !phases && t < b->Ns_max; ++t) {
@@ -825,11 +991,15 @@ int crys_batch_create(const crys_instance_t* desc, size_t count, int precision,
       b->cls[3 * i + a] = c;
     }
   }
-  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete b;
     return errf(CRYS_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
   }
+  b->phase_grid = batch_persistent_enabled() ? std::max(b->n_cu, 1) : 0;
   int r = precision == CRYS_FP32 ? batch_setup<float>(b) : batch_setup<double>(b);
   if (r != CRYS_OK) {
     batch_free(b);
@@ -932,7 +1102,32 @@ int crys_batch_get_path(const crys_batch_t* b, int* path) {
 int crys_batch_autotune(crys_batch_t* b, int reps, int* path, double ms[2]) {
   if (!b || reps <= 0) return errf(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
   double t_ph = -1.0, t_pa = -1.0;
-  if (b->phase_ok) BCHK(batch_time_steps(b, true, reps, &t_ph));
+  if (b->phase_ok) {
+    // the phases on one workgroup per item and on persistent grids of 1, 2
+    // and 4 workgroups per CU; the fastest grid is kept
+    const int keep = b->phase_grid;
+    const int grids[4] = {0, b->n_cu, 2 * b->n_cu, 4 * b->n_cu};
+    int best = keep;
+    for (int gi = 0; gi < 4; ++gi) {
+      if (gi > 0 && grids[gi] <= 0) continue;
+      b->phase_grid = grids[gi];
+      double t = -1.0;
+      const int r = batch_time_steps(b, true, reps, &t);
+      if (r != CRYS_OK) {
+        b->phase_grid = keep;
+        return r;
+      }
+      if (t_ph < 0 || t < t_ph) {
+        t_ph = t;
+        best = grids[gi];
+      }
+    }
+    if (best != keep) {
+      BHIP(hipStreamSynchronize(b->stream));
+      batch_drop_graph(b);
+    }
+    b->phase_grid = best;
+  }
   BCHK(batch_time_steps(b, false, reps, &t_pa));
   const bool ph = b->phase_ok && t_ph < t_pa;
   BCHK(crys_batch_set_path(b, ph ? CRYS_BATCH_PATH_PHASES : CRYS_BATCH_PATH_PASSES));
@@ -941,6 +1136,22 @@ int crys_batch_autotune(crys_batch_t* b, int reps, int* path, double ms[2]) {
     ms[0] = t_ph;
     ms[1] = t_pa;
   }
+  return CRYS_OK;
+}
+
+int crys_batch_set_phase_grid(crys_batch_t* b, int workgroups) {
+  if (!b || workgroups < 0) return errf(CRYS_ERR_INVALID_ARGUMENT, "bad arguments");
+  if (workgroups != b->phase_grid) {
+    BHIP(hipStreamSynchronize(b->stream));
+    batch_drop_graph(b);
+    b->phase_grid = workgroups;
+  }
+  return CRYS_OK;
+}
+
+int crys_batch_get_phase_grid(const crys_batch_t* b, int* workgroups) {
+  if (!b || !workgroups) return errf(CRYS_ERR_INVALID_ARGUMENT, "null argument");
+  *workgroups = b->phase_grid;
   return CRYS_OK;
 }
 

@@ -182,22 +182,39 @@ __device__ __forceinline__ void phase_col_product(
   }
 }
 
+// No work after the staging barrier (the launch-per-plane kernels).
+struct NoStagingHook {
+  __device__ void operator()() const {}
+};
+
 // One phase of one chain on one plane (base: its first element; RS: its row
 // stride), for the single-field and the batched kernels.  `lds` holds at
 // least MP * PhaseShape::PB elements.
-template <typename T, int MP, bool CEN, int NT_>
+//
+// With RAW the plane is not read from ch.in: a persistent kernel has already
+// moved it into LDS as compact rows (`raw`, element (r, c) at r * MC + c, see
+// dma_plane), and `after_staging` runs once the plane has been staged, when
+// `raw` may be overwritten (the persistent kernel starts the next plane's copy
+// there).
+template <typename T, int MP, bool CEN, int NT_, bool RAW = false,
+          typename Hook = NoStagingHook>
 __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const T* mat_r,
                                                  const T* mat_c,
                                                  const unsigned short* mat_r_lo,
                                                  const unsigned short* mat_c_lo, int MR, int MC,
                                                  long base, long RS, T* lds,
-                                                 bool shift = false) {
+                                                 bool shift = false, const T* raw = nullptr,
+                                                 const Hook& after_staging = Hook{},
+                                                 int tid_in = -1) {
   using S = PhaseShape<T, MP, CEN, NT_>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int NT = S::NT, JT = S::JT, H = S::H, PB = S::PB, PA = S::PA, TPW = S::TPW;
   constexpr bool SPLIT = std::is_same<T, float>::value;
 
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // a persistent caller passes the thread index through an opaque copy, so
+  // that nothing derived from it is hoisted out of its loop over planes and
+  // held in registers across the planes
+  const int tid = tid_in >= 0 ? tid_in : (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
   const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
   const bool lo_r = SPLIT && mat_r_lo != nullptr, lo_c = SPLIT && mat_c_lo != nullptr;
 
@@ -229,14 +246,29 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
   // a start phase), so that staging waits only for the plane; the other E of
   // a block junction is read in the E/history stage
   const T* e_a = fin ? ch.e_fin : ch.e_next;
+  if constexpr (RAW) {
+    // E from global memory first, then the plane from its LDS copy
 #pragma unroll
-  for (int k = 0; k < KE; ++k)
+    for (int k = 0; k < KE; ++k)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) x[k][v] = ok[k] ? ch.in[g[k][v]] : T(0);
+      for (int v = 0; v < NV; ++v) ef[k][v] = ok[k] ? e_a[g[k][v]] : T(0);
 #pragma unroll
-  for (int k = 0; k < KE; ++k)
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int r = f / MP, c = f - r * MP;
+      x[k][0] = ok[k] ? raw[r * MC + c] : T(0);
+      if constexpr (CEN) x[k][1] = ok[k] ? raw[cen_partner<MP>(r, shift) * MC + c] : T(0);
+    }
+  } else {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) ef[k][v] = ok[k] ? e_a[g[k][v]] : T(0);
+    for (int k = 0; k < KE; ++k)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) x[k][v] = ok[k] ? ch.in[g[k][v]] : T(0);
+#pragma unroll
+    for (int k = 0; k < KE; ++k)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) ef[k][v] = ok[k] ? e_a[g[k][v]] : T(0);
+  }
 
   // ---- stage the plane (start: times E of the next step) ----
 #pragma unroll
@@ -255,6 +287,7 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
     }
   }
   __syncthreads();
+  if constexpr (RAW) after_staging();
 
   V4 acc[TPW];
   const int jt = w % JT;

@@ -264,6 +264,15 @@ CRYS_API int crys_batch_get_path(const crys_batch_t* b, int* path);
  * timed steps write only the histories and scratch fields, so the batch's
  * fields and trace are unchanged. */
 CRYS_API int crys_batch_autotune(crys_batch_t* b, int reps, int* path, double ms[2]);
+/* The launch shape of the phases: 0 runs one workgroup per (instance, chain,
+ * plane) item; G > 0 runs a persistent grid of min(G, items) workgroups, each
+ * taking every G-th item and copying the next item's plane into LDS while it
+ * computes the current one (the default, with G = the device's compute
+ * units).  crys_batch_autotune times 0 and 1, 2 and 4 workgroups per CU and
+ * keeps the fastest; ms[0] is then the fastest phases' time.  No counterpart
+ * in the reference.  Results are identical for every G. */
+CRYS_API int crys_batch_set_phase_grid(crys_batch_t* b, int workgroups);
+CRYS_API int crys_batch_get_phase_grid(const crys_batch_t* b, int* workgroups);
 CRYS_API int crys_batch_get_info(const crys_batch_t* b, size_t i, size_t* n_phy, size_t* NsA,
                                  size_t* NsB);
 
@@ -351,6 +360,28 @@ CRYS_API int crys_scft_time_propagation(crys_scft_t* h, int reps, double* ms_per
                                         int* launches_per_step);
 CRYS_API int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step,
                                  double* ms_per_launch, int* nkernels);
+
+/* Synchronisation between contour steps, measured.  No counterpart in the
+ * reference.  Every phase of the one-launch contour step ends with all
+ * workgroups' planes written and starts by reading planes written on the
+ * other XCDs; crys_time_grid_sync times that exchange (each of `workgroups`
+ * workgroups of 1024 threads, one per CU, writes 64 KB, then reads the 64 KB
+ * that the workgroup 37 ids away wrote) for `rounds` rounds, synchronised by
+ * one of: */
+enum crys_sync_variant {
+  CRYS_SYNC_LAUNCH = 0,    /* a kernel launch per round (what the iteration uses) */
+  CRYS_SYNC_EMPTY = 1,     /* empty launches of the same shape, no exchange */
+  CRYS_SYNC_FENCED = 2,    /* persistent kernel, one counter, agent-scope fences */
+  CRYS_SYNC_TWO_LEVEL = 3, /* persistent, per-XCD counters then a root, agent-scope fences */
+  CRYS_SYNC_BYPASS = 4,    /* persistent, one counter, exchange with agent-scope accesses */
+  CRYS_SYNC_BARRIER = 5    /* persistent, FENCED without the exchange */
+};
+/* Writes the mean microseconds per round, the number of exchanged values a
+ * reader found not yet written (must be 0) and the number of bounded spins
+ * that timed out (must be 0; a time-out ends every workgroup).  8..256
+ * workgroups, a multiple of 8.  Allocates its own buffers. */
+CRYS_API int crys_time_grid_sync(int variant, int workgroups, int rounds, double* us_per_round,
+                                 unsigned* not_visible, unsigned* timeouts);
 
 #ifdef __cplusplus
 }

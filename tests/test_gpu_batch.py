@@ -196,3 +196,23 @@ def test_batch_path_rules():
     small.set_path("auto")
     assert small.path() == "phases"
     small.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_phase_grids_are_bitwise_identical(prec):
+    """The persistent phases (grids of 7 workgroups, one and two per CU) and the
+    one-workgroup-per-item phases run the same body on the same items: the
+    traces and fields agree bit for bit."""
+    out = {}
+    for grid in (0, 7, 256, 512):
+        b = Batch(PHASE_BATCH, precision=prec)
+        for k, inst in enumerate(PHASE_BATCH):
+            b.set_fields(k, *seeded_fields(inst, 400 + k))
+        b.set_path("phases")
+        b.set_phase_grid(grid)
+        assert b.phase_grid() == grid
+        out[grid] = (b.run(3), b.fields(5)[0], b.fields(8)[2])
+        b.close()
+    for grid in (7, 256, 512):
+        for x, y in zip(out[grid], out[0]):
+            np.testing.assert_array_equal(x, y)

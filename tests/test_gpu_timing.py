@@ -79,3 +79,23 @@ def test_diffusion_time_agrees_with_the_step_kernels():
     assert len(d["per_launch"]) == 2
     assert 0.8 * d["diffusion"] <= sum(d["per_launch"]) <= 4 * d["diffusion"]
     assert 0.8 * d["diffusion"] <= d["step"] <= 4 * d["diffusion"]
+
+
+@pytest.mark.parametrize("workgroups", [64, 256])
+def test_grid_sync_variants_exchange_correctly(workgroups):
+    """Every synchronisation variant of crys_time_grid_sync delivers the
+    exchanged values (none read before they were written) without a spin
+    time-out, and reports a positive time per round."""
+    import ctypes
+
+    from crysscft_sample_amd._lib import check, lib
+
+    for v in range(6):
+        us = ctypes.c_double()
+        bad, tmo = ctypes.c_uint(), ctypes.c_uint()
+        check(lib().crys_time_grid_sync(v, workgroups, 50, ctypes.byref(us), ctypes.byref(bad),
+                                        ctypes.byref(tmo)))
+        print(f"  variant {v}, {workgroups} workgroups: {us.value:.2f} us per round", flush=True,
+              file=sys.stderr)
+        assert bad.value == 0 and tmo.value == 0
+        assert 0 < us.value < 1e4
