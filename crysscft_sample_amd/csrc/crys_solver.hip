@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <array>
 #include <new>
 #include <string>
 #include <vector>
@@ -131,6 +132,9 @@ struct OpSet {
   void* dmat_c_lo[3];
   void* dmat_dL[3];  // dD/dL per axis, built on first use (cell stress)
   void* drad[3];     // radix-pass tables (radix_pass.hpp) when the radix passes are selected
+  // 3m with translational parts: the separable terms of the folded operator
+  // (crys::m3_glide_terms), per term and axis; their L derivatives on first use
+  std::vector<std::array<void*, 3>> gmat, glo, gdL;
 };
 
 struct crys_solver {
@@ -150,20 +154,17 @@ struct crys_solver {
   // bumped whenever the operator set in use changes (new cell, new or other
   // cached ds): a captured iteration graph records the operators' addresses
   unsigned long ops_gen = 0;
-  // 3m with non-zero translational parts: diffusion() expands the reduced
-  // field to the logical grid, applies the P1 operators of `full` (a P1
-  // solver on this solver's stream) and extracts the even sub-lattice
-  struct crys_solver* full = nullptr;
-  bool own_stream = true;
+  // 3m with non-zero translational parts: diffusion() is the sum of the
+  // separable terms of OpSet::gmat on the reduced grid; glide_acc and
+  // glide_tmp (n_phy elements of the precision) hold the partial sums
+  bool glide = false;
   crys::M3Ops m3{};
+  void* glide_acc = nullptr;
+  void* glide_tmp = nullptr;
 };
 
 struct crys_scft {
-  crys_solver* sv = nullptr;
-  // a 3m solver with translational parts: the iteration runs on sv (its P1
-  // solver) with the fields expanded; m3 is the 3m solver the caller passed
-  const crys_solver* m3 = nullptr;
-  crys_solver* user = nullptr;  // the solver the caller passed (sv, or the 3m solver)
+  crys_solver* sv = nullptr;  // the solver the caller passed (not owned)
   double fA = 0, fB = 0, ds = 0, xN = 0, acc = 0, L[3] = {0, 0, 0};
   size_t NsA = 0, NsB = 0, Ns = 0, n = 0;
   double *wA = nullptr, *wB = nullptr, *ksi = nullptr, *phiA = nullptr, *phiB = nullptr;
@@ -198,6 +199,11 @@ void free_ops(crys_solver* h) {
       if (o.dmat_dL[a]) (void)hipFree(o.dmat_dL[a]);
       if (o.drad[a]) (void)hipFree(o.drad[a]);
     }
+  for (auto& o : h->ops)
+    for (auto* v : {&o.gmat, &o.glo, &o.gdL})
+      for (auto& t : *v)
+        for (void* m : t)
+          if (m) (void)hipFree(m);
   h->ops.clear();
   h->cur = -1;
   ++h->ops_gen;
@@ -264,11 +270,29 @@ bool radix_axis(const crys_solver* h, int a) {
   }
 }
 
+// The separable terms of a 3m solver with translational parts
+// (crys::m3_glide_terms) on the device: values (and, for FP32, their bf16 low
+// parts) or, with dL, their L derivatives.
+int upload_glide_terms(crys_solver* h, double ds, bool dL, OpSet* o) {
+  int N[3];
+  for (int a = 0; a < 3; ++a) N[a] = (int)h->nlog[a];
+  const std::vector<crys::M3Term> terms =
+      crys::m3_glide_terms(h->m3.mask, h->m3.shift, N, h->cell, ds, dL);
+  auto& dst = dL ? o->gdL : o->gmat;
+  dst.assign(terms.size(), {nullptr, nullptr, nullptr});
+  if (!dL) o->glo.assign(terms.size(), {nullptr, nullptr, nullptr});
+  for (size_t t = 0; t < terms.size(); ++t)
+    for (int a = 0; a < 3; ++a)
+      CHK(upload_matrix(h, terms[t].At[a], &dst[t][a], dL ? nullptr : &o->glo[t][a]));
+  return CRYS_OK;
+}
+
 int build_ops(crys_solver* h, double ds, OpSet* out) {
   out->ds = ds;
   for (int a = 0; a < 3; ++a)
     out->dmat[a] = out->dmat_lo[a] = out->dmat_c[a] = out->dmat_c_lo[a] = out->dmat_dL[a] =
         out->drad[a] = nullptr;
+  if (h->glide) return upload_glide_terms(h, ds, false, out);
   CHK(upload_axis_ops(h, ds, false, out->dmat, out->dmat_lo, out->dmat_c, out->dmat_c_lo));
   for (int a = 0; a < 3; ++a) {
     if (!radix_axis(h, a)) continue;
@@ -367,12 +391,66 @@ int launch_step(const crys_solver* h, const OpSet& op, int nch, const T* const i
   return CRYS_OK;
 }
 
+// One contour step of a 3m solver with translational parts for 1 or 2
+// chains: out = E_post (.) sum_t X_t (x) Y_t (x) Z_t (E_pre (.) in) over the
+// separable terms of OpSet::gmat.  Term t is a z pass (in -> tmp, times
+// E_pre), a y pass in place and an x pass; a single term writes out directly,
+// else the first writes acc, each further one tmp2, summed into acc and, for
+// the last term, into out (times E_post).  in may equal out: every term reads
+// in before out is written.  E_pre, E_post or their entries may be null.
+// dL_axis >= 0 applies the L derivative of every term's factor on that axis
+// (the stress).
+template <typename T>
+int launch_glide_step(const crys_solver* h, const OpSet& op, int nch, const T* const in[2],
+                      T* const out[2], T* const tmp[2], T* const acc[2], T* const tmp2[2],
+                      const T* const E_pre[2], const T* const E_post[2], hipStream_t s,
+                      bool stream, int dL_axis = -1) {
+  crys::PassParams<T> p{};
+  p.X = (int)h->nphy[0];
+  p.Y = (int)h->nphy[1];
+  p.Z = (int)h->nphy[2];
+  p.stream = stream;
+  const size_t nt = op.gmat.size();
+  const size_t n = h->n_phy;
+  if (nt == 0 || (dL_axis >= 0 && op.gdL.size() != nt))
+    return fail(CRYS_ERR_HIP, "3m operator terms are missing");
+  for (size_t t = 0; t < nt; ++t) {
+    const bool last = t + 1 == nt;
+    for (int k = 0; k < 3; ++k) {
+      const int ax = 2 - k;  // z, y, x
+      for (int c = 0; c < nch; ++c) {
+        T* xo = nt == 1 ? out[c] : (t == 0 ? acc[c] : tmp2[c]);
+        const T* ep = E_pre ? E_pre[c] : nullptr;
+        const T* eq = (nt == 1 && E_post) ? E_post[c] : nullptr;
+        p.c[c] = k == 0   ? crys::PassChain<T>{in[c], tmp[c], ep, nullptr}
+                 : k == 1 ? crys::PassChain<T>{tmp[c], tmp[c], nullptr, nullptr}
+                          : crys::PassChain<T>{tmp[c], xo, nullptr, eq};
+      }
+      const bool d = ax == dL_axis;
+      p.mat = (const T*)(d ? op.gdL[t][ax] : op.gmat[t][ax]);
+      p.mat_lo = d ? nullptr : (const unsigned short*)op.glo[t][ax];
+      p.cen = 0;
+      HIPCHK(crys::launch_axis_pass<T>(ax, p, nch, s));
+    }
+    if (t >= 1) {
+      crys::GlideSum<T> g{};
+      for (int c = 0; c < nch; ++c)
+        g.c[c] = {last ? out[c] : acc[c], acc[c], tmp2[c],
+                  (last && E_post) ? E_post[c] : nullptr};
+      hipLaunchKernelGGL((crys::glide_sum_kernel<T>), dim3(grid_for(n), (unsigned)nch), dim3(256),
+                         0, s, g, n);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  return CRYS_OK;
+}
+
 // Length class of the one-launch phase path for this solver (0: not used):
 // the matrix-core plane phases when all three axes share a length class up
 // to 64, the radix plane phase when all three axes have length 128 and take
 // the radix pass.
 int phase_mp(const crys_solver* h) {
-  if (!crys::phase_enabled()) return 0;
+  if (!crys::phase_enabled() || h->glide) return 0;
   if (h->nphy[0] == 128 && h->nphy[1] == 128 && h->nphy[2] == 128 && radix_axis(h, 0) &&
       radix_axis(h, 1) && radix_axis(h, 2))
     return 128;
@@ -447,20 +525,6 @@ int check_handle(const crys_solver* h) {
 }
 
 int diffusion_impl(crys_solver* h) {
-  if (h->full) {
-    // 3m with translational parts: expand, P1 on the logical grid, extract
-    if (h->full->cur < 0)
-      return fail(CRYS_ERR_NO_CONTOUR_STEP, "diffusion() before set_contour_step()");
-    const int X = (int)h->nphy[0], Y = (int)h->nphy[1], Z = (int)h->nphy[2];
-    hipLaunchKernelGGL(crys::m3_expand_kernel, dim3(grid_for(h->n_phy)), dim3(256), 0, h->stream,
-                       (const double*)h->iomat, h->full->iomat, h->m3, X, Y, Z);
-    HIPCHK(hipGetLastError());
-    CHK(diffusion_impl(h->full));
-    hipLaunchKernelGGL(crys::m3_extract_kernel, dim3(grid_for(h->n_phy)), dim3(256), 0, h->stream,
-                       (const double*)h->full->iomat, h->iomat, X, Y, Z, 1.0);
-    HIPCHK(hipGetLastError());
-    return CRYS_OK;
-  }
   if (h->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "diffusion() before set_contour_step()");
   const OpSet& op = h->ops[h->cur];
   const size_t n = h->n_phy;
@@ -470,6 +534,13 @@ int diffusion_impl(crys_solver* h) {
     double* out[2] = {io, nullptr};
     double* scr[2] = {io, nullptr};
     const double* E[2] = {nullptr, nullptr};
+    if (h->glide) {
+      double* tmp[2] = {(double*)h->scratch, nullptr};
+      double* acc[2] = {(double*)h->glide_acc, nullptr};
+      double* tmp2[2] = {(double*)h->glide_tmp, nullptr};
+      return launch_glide_step<double>(h, op, 1, in, out, tmp, acc, tmp2, nullptr, nullptr,
+                                       h->stream, false);
+    }
     return launch_step<double>(h, op, 1, in, out, scr, E, h->stream, false);
   }
   float* w = (float*)h->work;
@@ -480,7 +551,15 @@ int diffusion_impl(crys_solver* h) {
   float* out[2] = {w, nullptr};
   float* scr[2] = {w, nullptr};
   const float* E[2] = {nullptr, nullptr};
-  CHK(launch_step<float>(h, op, 1, in, out, scr, E, h->stream, false));
+  if (h->glide) {
+    float* tmp[2] = {(float*)h->scratch, nullptr};
+    float* acc[2] = {(float*)h->glide_acc, nullptr};
+    float* tmp2[2] = {(float*)h->glide_tmp, nullptr};
+    CHK(launch_glide_step<float>(h, op, 1, in, out, tmp, acc, tmp2, nullptr, nullptr, h->stream,
+                                 false));
+  } else {
+    CHK(launch_step<float>(h, op, 1, in, out, scr, E, h->stream, false));
+  }
   hipLaunchKernelGGL((crys::convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0,
                      h->stream, h->iomat, (const float*)w, n);
   HIPCHK(hipGetLastError());
@@ -502,28 +581,6 @@ int propagate_impl(crys_solver_t* h, const double* E, const double* init, size_t
   if (h->precision != CRYS_FP64)
     return fail(CRYS_ERR_UNSUPPORTED, "crys_propagate takes FP64 histories; use an FP64 solver");
   const size_t n = h->n_phy;
-  if (h->full) {
-    // 3m with translational parts: E, diffusion() on the iomat, E per step
-    if (h->full->cur < 0)
-      return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
-    size_t s = direction == 1 ? 0 : Ns;
-    HIPCHK(hipMemcpyAsync(q + s * n, init, n * sizeof(double), hipMemcpyDeviceToDevice,
-                          h->stream));
-    for (size_t it = 0; it < Ns; ++it) {
-      const size_t sn = direction == 1 ? s + 1 : s - 1;
-      hipLaunchKernelGGL(crys::mul_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream,
-                         (const double*)(q + s * n), E, h->iomat, n);
-      HIPCHK(hipGetLastError());
-      CHK(diffusion_impl(h));
-      hipLaunchKernelGGL(crys::mul_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream,
-                         (const double*)h->iomat, E, q + sn * n, n);
-      HIPCHK(hipGetLastError());
-      s = sn;
-    }
-    HIPCHK(hipMemcpyAsync(h->iomat, q + s * n, n * sizeof(double), hipMemcpyDeviceToDevice,
-                          h->stream));
-    return CRYS_OK;
-  }
   if (h->cur < 0) return fail(CRYS_ERR_NO_CONTOUR_STEP, "propagate before set_contour_step()");
   const OpSet& op = h->ops[h->cur];
   size_t s = direction == 1 ? 0 : Ns;
@@ -534,7 +591,13 @@ int propagate_impl(crys_solver_t* h, const double* E, const double* init, size_t
     double* out[2] = {q + sn * n, nullptr};
     double* sc[2] = {scr, nullptr};
     const double* Ee[2] = {E, nullptr};
-    CHK(launch_step<double>(h, op, 1, in, out, sc, Ee, h->stream, true));
+    if (h->glide) {
+      double* acc[2] = {(double*)h->glide_acc, nullptr};
+      double* tmp2[2] = {(double*)h->glide_tmp, nullptr};
+      CHK(launch_glide_step<double>(h, op, 1, in, out, sc, acc, tmp2, Ee, Ee, h->stream, true));
+    } else {
+      CHK(launch_step<double>(h, op, 1, in, out, sc, Ee, h->stream, true));
+    }
     s = sn;
   }
   HIPCHK(hipMemcpyAsync(h->iomat, q + s * n, n * sizeof(double), hipMemcpyDeviceToDevice,
@@ -668,26 +731,14 @@ int crys_solver_create(int kind, const size_t grid_log[3], const double translat
   if (e == hipSuccess) e = hipMalloc(&h->iomat, h->n_phy * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->work, h->n_phy * elem_size(precision));
   if (e == hipSuccess) e = hipMalloc(&h->scratch, h->n_phy * elem_size(precision));
+  h->glide = glide;
+  if (glide && e == hipSuccess) e = hipMalloc(&h->glide_acc, h->n_phy * elem_size(precision));
+  if (glide && e == hipSuccess) e = hipMalloc(&h->glide_tmp, h->n_phy * elem_size(precision));
   if (e == hipSuccess) e = hipMemset(h->iomat, 0, h->n_phy * sizeof(double));
   if (e != hipSuccess) {
     crys_solver_destroy(h);
     return fail(e == hipErrorOutOfMemory ? CRYS_ERR_OUT_OF_MEMORY : CRYS_ERR_HIP,
                 "solver allocation failed: %s", hipGetErrorString(e));
-  }
-  if (glide) {
-    crys_solver* p1 = nullptr;
-    const int r = crys_solver_create(CRYS_P1, grid_log, nullptr, precision, 0, &p1);
-    if (r != CRYS_OK) {
-      std::string keep = g_err;
-      crys_solver_destroy(h);
-      g_err = keep;
-      return r;
-    }
-    (void)hipStreamSynchronize(p1->stream);
-    (void)hipStreamDestroy(p1->stream);
-    p1->stream = h->stream;
-    p1->own_stream = false;
-    h->full = p1;
   }
   *out = h;
   return CRYS_OK;
@@ -696,12 +747,13 @@ int crys_solver_create(int kind, const size_t grid_log[3], const double translat
 int crys_solver_destroy(crys_solver_t* h) {
   if (!h) return CRYS_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  if (h->full) crys_solver_destroy(h->full);  // borrows this stream
   free_ops(h);
+  if (h->glide_acc) (void)hipFree(h->glide_acc);
+  if (h->glide_tmp) (void)hipFree(h->glide_tmp);
   if (h->iomat) (void)hipFree(h->iomat);
   if (h->work) (void)hipFree(h->work);
   if (h->scratch) (void)hipFree(h->scratch);
-  if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CRYS_OK;
 }
@@ -727,18 +779,12 @@ int crys_set_cell_para(crys_solver_t* h, const double c[6]) {
   }
   free_ops(h);
   h->ds_current = 0.0;
-  if (h->full) CHK(crys_set_cell_para(h->full, h->cell));
   return CRYS_OK;
 }
 
 int crys_set_contour_step(crys_solver_t* h, double ds) {
   CHK(check_handle(h));
   if (!(ds > 0) || !std::isfinite(ds)) return fail(CRYS_ERR_INVALID_ARGUMENT, "ds must be > 0");
-  if (h->full) {  // 3m with translational parts: the P1 solver holds the operators
-    CHK(crys_set_contour_step(h->full, ds));
-    h->ds_current = ds;
-    return CRYS_OK;
-  }
   if (!h->ops.empty() && ds == h->ds_current && h->cur >= 0) return CRYS_OK;
   h->ds_current = ds;
   for (size_t i = 0; i < h->ops.size(); ++i)
@@ -757,6 +803,10 @@ int crys_set_contour_step(crys_solver_t* h, double ds) {
       if (o.dmat_c_lo[a]) (void)hipFree(o.dmat_c_lo[a]);
       if (o.drad[a]) (void)hipFree(o.drad[a]);
     }
+    for (auto* v : {&o.gmat, &o.glo})
+      for (auto& t : *v)
+        for (void* m : t)
+          if (m) (void)hipFree(m);
     return r;
   }
   h->ops.push_back(o);
@@ -930,12 +980,18 @@ int propagate_segment(crys_scft* h, hipStream_t s, int nch, T* qf, T* qb, size_t
     for (size_t k = 0; k <= c; ++k)
       CHK(launch_phase_step<T>(sv, op, k, c, NsA, qf, qb, scr, EA, EB, s, nch, t0));
   } else {
+    // a 3m solver with translational parts: scr holds 3 n per chain
+    T* acc[2] = {scr[0] + n, scr[1] + n};
+    T* tmp2[2] = {scr[0] + 2 * n, scr[1] + 2 * n};
     for (size_t t = 0; t < c; ++t) {
       const size_t bo = c - t - 1;
       const T* in[2] = {qf + t * n, qb + (c - t) * n};
       T* out[2] = {qf + (t + 1) * n, qb + bo * n};
       const T* E[2] = {t0 + t < NsA ? EA : EB, t0 + bo >= NsA ? EB : EA};
-      CHK(launch_step<T>(sv, op, nch, in, out, scr, E, s, true));
+      if (sv->glide)
+        CHK(launch_glide_step<T>(sv, op, nch, in, out, scr, acc, tmp2, E, E, s, true));
+      else
+        CHK(launch_step<T>(sv, op, nch, in, out, scr, E, s, true));
     }
   }
   return CRYS_OK;
@@ -961,6 +1017,22 @@ int scft_record_stress(crys_scft* h, hipStream_t s, const T* qf, const T* qb, si
     const T* E = (const T*)(t < NsA ? h->EA : h->EB);
     const T* q = qf + (t - slot0) * n;
     const T* qd = qb + (t + 1 - slot0) * n;
+    if (sv->glide) {
+      // the derivative of every separable term on axis a, summed (scr[0]
+      // holds 3 n: the term in flight, the sum, the next term)
+      T* tmp[2] = {w, nullptr};
+      T* acc[2] = {w + n, nullptr};
+      T* tmp2[2] = {w + 2 * n, nullptr};
+      const T* in[2] = {q, nullptr};
+      const T* Ep[2] = {E, nullptr};
+      for (int a = 0; a < 3; ++a) {
+        CHK(launch_glide_step<T>(sv, op, 1, in, acc, tmp, acc, tmp2, Ep, nullptr, s, false, a));
+        hipLaunchKernelGGL((crys::dot3_partial_kernel<T>), dim3(nb), dim3(crys::kRedThreads), 0,
+                           s, (const T*)(w + n), E, qd, h->spart + (3 * t + a) * nb, n);
+        HIPCHK(hipGetLastError());
+      }
+      continue;
+    }
     for (int a = 0; a < 3; ++a) {
       // z pass (pre E), then y, then x; the axis a pass uses dD/dL_a
       const int order[3] = {2, 1, 0};
@@ -1168,7 +1240,7 @@ int scft_build_graph(crys_scft* h) {
 int scft_ensure_ops(crys_scft* h) {
   const crys_solver* sv = h->sv;
   if (sv->cur >= 0 && sv->ops[sv->cur].ds == h->ds) return CRYS_OK;
-  return crys_set_contour_step(h->user, h->ds);
+  return crys_set_contour_step(h->sv, h->ds);
 }
 
 template <typename T>
@@ -1178,23 +1250,12 @@ void scft_prep(crys_scft* h) {
                      (T*)h->EB, h->xN, h->ds, h->n);
 }
 
-// Device fields in the solver's physical layout: copied (3m with
-// translational parts: expanded to the logical grid) into the handle's
+// Device fields in the solver's physical layout: copied into the handle's
 // fields, then calc_ksi / array_exp.  Asynchronous on the solver's stream.
 int scft_set_fields_device(crys_scft* h, const double* wA, const double* wB) {
   hipStream_t s = h->sv->stream;
-  if (h->m3) {
-    const int X = (int)h->m3->nphy[0], Y = (int)h->m3->nphy[1], Z = (int)h->m3->nphy[2];
-    const unsigned g = grid_for(h->m3->n_phy);
-    hipLaunchKernelGGL(crys::m3_expand_kernel, dim3(g), dim3(256), 0, s, wA, h->wA, h->m3->m3, X,
-                       Y, Z);
-    hipLaunchKernelGGL(crys::m3_expand_kernel, dim3(g), dim3(256), 0, s, wB, h->wB, h->m3->m3, X,
-                       Y, Z);
-    HIPCHK(hipGetLastError());
-  } else {
-    HIPCHK(hipMemcpyAsync(h->wA, wA, h->n * 8, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(h->wB, wB, h->n * 8, hipMemcpyDeviceToDevice, s));
-  }
+  HIPCHK(hipMemcpyAsync(h->wA, wA, h->n * 8, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(h->wB, wB, h->n * 8, hipMemcpyDeviceToDevice, s));
   if (h->sv->precision == CRYS_FP32) scft_prep<float>(h);
   else scft_prep<double>(h);
   HIPCHK(hipGetLastError());
@@ -1225,7 +1286,11 @@ int scft_evaluate_impl(crys_scft* h, bool device, const double* wA, const double
   if (!h->sums) HIPCHK(hipMalloc((void**)&h->sums, (1 + 3 * Ns) * 8));
   if (stress) {
     if (!h->spart) HIPCHK(hipMalloc((void**)&h->spart, 3 * Ns * nb * 8));
-    if (!op.dmat_dL[0]) CHK(upload_axis_ops(sv, op.ds, true, op.dmat_dL));
+    if (sv->glide) {
+      if (op.gdL.empty()) CHK(upload_glide_terms(sv, op.ds, true, &op));
+    } else if (!op.dmat_dL[0]) {
+      CHK(upload_axis_ops(sv, op.ds, true, op.dmat_dL));
+    }
   }
   hipLaunchKernelGGL(crys::set_int_kernel, dim3(1), dim3(1), 0, s, h->counter, 0);
   HIPCHK(hipGetLastError());
@@ -1250,45 +1315,14 @@ int scft_evaluate_impl(crys_scft* h, bool device, const double* wA, const double
   HIPCHK(hipMemcpyAsync(row, h->trace, sizeof(row), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(sums.data(), h->sums, (stress ? 1 + 3 * Ns : 1) * 8,
                         hipMemcpyDeviceToHost, s));
-  // a reduced value of a 3m solver with translational parts sets its 8
-  // images, which share one gradient: 8 times the full-grid gradient
   if (device) {
-    double* out[2] = {dH_dwA, dH_dwB};
-    const double* g[2] = {h->gA, h->gB};
-    for (int k = 0; k < 2; ++k) {
-      if (!out[k]) continue;
-      if (h->m3) {
-        const int X = (int)h->m3->nphy[0], Y = (int)h->m3->nphy[1], Z = (int)h->m3->nphy[2];
-        hipLaunchKernelGGL(crys::m3_extract_kernel, dim3(grid_for(h->m3->n_phy)), dim3(256), 0, s,
-                           g[k], out[k], X, Y, Z, 8.0);
-        HIPCHK(hipGetLastError());
-      } else {
-        HIPCHK(hipMemcpyAsync(out[k], g[k], n * 8, hipMemcpyDeviceToDevice, s));
-      }
-    }
-    HIPCHK(hipStreamSynchronize(s));
+    if (dH_dwA) HIPCHK(hipMemcpyAsync(dH_dwA, h->gA, n * 8, hipMemcpyDeviceToDevice, s));
+    if (dH_dwB) HIPCHK(hipMemcpyAsync(dH_dwB, h->gB, n * 8, hipMemcpyDeviceToDevice, s));
   } else {
-    std::vector<double> gfa, gfb;
-    double* outA = dH_dwA;
-    double* outB = dH_dwB;
-    if (h->m3) {  // gradients on the full grid first
-      if (dH_dwA) {
-        gfa.resize(n);
-        outA = gfa.data();
-      }
-      if (dH_dwB) {
-        gfb.resize(n);
-        outB = gfb.data();
-      }
-    }
-    if (outA) HIPCHK(hipMemcpyAsync(outA, h->gA, n * 8, hipMemcpyDeviceToHost, s));
-    if (outB) HIPCHK(hipMemcpyAsync(outB, h->gB, n * 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (h->m3) {
-      if (dH_dwA) crys::m3_extract_host(gfa.data(), dH_dwA, h->m3->nphy, 8.0);
-      if (dH_dwB) crys::m3_extract_host(gfb.data(), dH_dwB, h->m3->nphy, 8.0);
-    }
+    if (dH_dwA) HIPCHK(hipMemcpyAsync(dH_dwA, h->gA, n * 8, hipMemcpyDeviceToHost, s));
+    if (dH_dwB) HIPCHK(hipMemcpyAsync(dH_dwB, h->gB, n * 8, hipMemcpyDeviceToHost, s));
   }
+  HIPCHK(hipStreamSynchronize(s));
   const double Q = row[CRYS_TRACE_Q];
   if (trace_row) memcpy(trace_row, row, sizeof(row));
   if (H) *H = sums[0] / (double)n - log(Q);
@@ -1327,14 +1361,6 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
   crys_scft* h = new (std::nothrow) crys_scft();
   if (!h) return fail(CRYS_ERR_OUT_OF_MEMORY, "host allocation failed");
   h->sv = solver;
-  h->user = solver;
-  if (solver->full) {
-    // 3m with translational parts: iterate the expanded fields with the P1
-    // solver; every full-grid mean equals the reduced mean (each reduced point
-    // has 8 images), so the trace is the 3m trace
-    h->sv = solver->full;
-    h->m3 = solver;
-  }
   h->fA = fA;
   h->fB = fB;
   h->ds = ds;
@@ -1360,9 +1386,12 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
   A(&h->EA, n * es);
   A(&h->EB, n * es);
   // two fields per chain: the phases alternate between them (a phase may
-  // run on two workgroups per plane, so it cannot rewrite its input in place)
-  A(&h->scr[0], 2 * n * es);
-  A(&h->scr[1], 2 * n * es);
+  // run on two workgroups per plane, so it cannot rewrite its input in place);
+  // three for a 3m solver with translational parts (launch_glide_step: the
+  // term in flight, the partial sum and the next term)
+  const size_t nscr = solver->glide ? 3 : 2;
+  A(&h->scr[0], nscr * n * es);
+  A(&h->scr[1], nscr * n * es);
   A((void**)&h->qpart, crys::kMaxRedBlocks * 8);
   A((void**)&h->epart, 4 * crys::kMaxRedBlocks * 8);
   A((void**)&h->scal, 4 * 8);
@@ -1430,15 +1459,6 @@ int crys_scft_get_checkpointing(crys_scft_t* h, size_t* every, size_t* storage_b
 int crys_scft_set_fields(crys_scft_t* h, const double* wA, const double* wB) {
   if (!h || !wA || !wB) return fail(CRYS_ERR_INVALID_ARGUMENT, "null argument");
   hipStream_t s = h->sv->stream;
-  std::vector<double> fa, fb;
-  if (h->m3) {
-    fa.resize(h->n);
-    fb.resize(h->n);
-    crys::m3_expand_host(wA, fa.data(), h->m3->m3, h->m3->nphy);
-    crys::m3_expand_host(wB, fb.data(), h->m3->m3, h->m3->nphy);
-    wA = fa.data();
-    wB = fb.data();
-  }
   HIPCHK(hipMemcpyAsync(h->wA, wA, h->n * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(h->wB, wB, h->n * 8, hipMemcpyHostToDevice, s));
   if (h->sv->precision == CRYS_FP32) scft_prep<float>(h);
@@ -1483,18 +1503,6 @@ int crys_scft_get_fields(crys_scft_t* h, double* phiA, double* phiB, double* wA,
   if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
   hipStream_t s = h->sv->stream;
   const size_t b = h->n * 8;
-  if (h->m3) {  // full-grid fields, reduced to the 3m sub-lattice
-    std::vector<double> f(h->n);
-    const double* src[4] = {h->phiA, h->phiB, h->wA, h->wB};
-    double* dst[4] = {phiA, phiB, wA, wB};
-    for (int k = 0; k < 4; ++k) {
-      if (!dst[k]) continue;
-      HIPCHK(hipMemcpyAsync(f.data(), src[k], b, hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      crys::m3_extract_host(f.data(), dst[k], h->m3->nphy, 1.0);
-    }
-    return CRYS_OK;
-  }
   if (phiA) HIPCHK(hipMemcpyAsync(phiA, h->phiA, b, hipMemcpyDeviceToHost, s));
   if (phiB) HIPCHK(hipMemcpyAsync(phiB, h->phiB, b, hipMemcpyDeviceToHost, s));
   if (wA) HIPCHK(hipMemcpyAsync(wA, h->wA, b, hipMemcpyDeviceToHost, s));
@@ -1628,6 +1636,13 @@ int crys_scft_time_step(crys_scft_t* h, int reps, double* ms_per_step, double* m
       double* sc[2] = {(double*)h->scr[0], (double*)h->scr[1]};
       return launch_phase_step<double>(sv, op, 1, Ns, h->NsA, (double*)h->qf, (double*)h->qb, sc,
                                        (const double*)h->EA, (const double*)h->EB, s);
+    }
+    if (sv->glide) {
+      // the whole step of the separable terms, timed as one entry
+      nl = 1;
+      if (sv->precision == CRYS_FP32)
+        return propagate_segment<float>(h, s, 2, (float*)h->qf, (float*)h->qb, 1, 0);
+      return propagate_segment<double>(h, s, 2, (double*)h->qf, (double*)h->qb, 1, 0);
     }
     if (sv->precision == CRYS_FP32) {
       float* qf = (float*)h->qf;

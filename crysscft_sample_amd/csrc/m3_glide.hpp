@@ -3,12 +3,16 @@
 // solver_utilities.h:55-66).
 //
 // The reference folds the full-grid multiplier into 16 coefficient arrays
-// with a phase per symmetry operation.  Here the same linear map is applied
-// as: expand the reduced field (the even sub-lattice f[2x,2y,2z]) to the full
-// logical grid with the 8 operations of the group, apply the P1 operators
-// there, and take the even sub-lattice back.  This is exact: the diffusion
-// operator commutes with the operations, and the 8 operations map the even
-// sub-lattice one to one onto the grid (checked at solver creation).
+// with a phase per symmetry operation and works on the half grid in Fourier
+// space.  Here the same linear map is applied on the reduced grid in real
+// space: operation o maps the even sub-lattice point j of an axis to its
+// image on the logical grid, and the P1 operator restricted to
+// (sub-lattice row, image column) gives one factor per axis and operation;
+// the sum over the 8 operations of the tensor products of their factors, with
+// terms that share two factors merged (crys::m3_glide_terms in
+// operators.cpp), is the diffusion of the reduced field.  The 8 operations
+// must map the even sub-lattice one to one onto the grid (checked at solver
+// creation).
 //
 // Operation o mirrors the axes in mask[o] (bit 0 = x, 1 = y, 2 = z) and
 // translates by t_o (fractions of the cell).  The translations are those of
@@ -72,74 +76,26 @@ inline bool m3_nonzero(const double g[9]) {
   return false;
 }
 
-// image of reduced coordinate j on an axis of logical length n
-__host__ __device__ inline long m3_image(long j, int mirrored, int shift, long n) {
-  const long v = mirrored ? -1 - 2 * j + shift : 2 * j + shift;
-  return ((v % n) + n) % n;
-}
+// Sum of two terms of a contour step, per chain (blockIdx.y): out = (a + b)
+// times e where e is not null.
+template <typename T>
+struct GlideSum {
+  struct Chain {
+    T* out;
+    const T* a;
+    const T* b;
+    const T* e;
+  } c[2];
+};
 
-// full[op(2 r)] = red[r] for the 8 operations; X, Y, Z the reduced lengths.
-static __global__ void m3_expand_kernel(const double* __restrict__ red, double* __restrict__ full,
-                                        M3Ops m, int X, int Y, int Z) {
-  const size_t n = (size_t)X * Y * Z;
-  const long NX = 2 * X, NY = 2 * Y, NZ = 2 * Z;
+template <typename T>
+__global__ void glide_sum_kernel(GlideSum<T> g, size_t n) {
+  const typename GlideSum<T>::Chain ch = g.c[blockIdx.y];
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    const long x = (long)(i / ((size_t)Y * Z)), y = (long)((i / Z) % Y), z = (long)(i % Z);
-    const double v = red[i];
-#pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      const long fx = m3_image(x, m.mask[o] & 1, m.shift[o][0], NX);
-      const long fy = m3_image(y, (m.mask[o] >> 1) & 1, m.shift[o][1], NY);
-      const long fz = m3_image(z, (m.mask[o] >> 2) & 1, m.shift[o][2], NZ);
-      full[((size_t)fx * NY + fy) * NZ + fz] = v;
-    }
+    const T v = ch.a[i] + ch.b[i];
+    ch.out[i] = ch.e ? v * ch.e[i] : v;
   }
-}
-
-// red[r] = scale * full[2 r]
-static __global__ void m3_extract_kernel(const double* __restrict__ full, double* __restrict__ red,
-                                         int X, int Y, int Z, double scale) {
-  const size_t n = (size_t)X * Y * Z;
-  const size_t NY = 2 * (size_t)Y, NZ = 2 * (size_t)Z;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const size_t x = i / ((size_t)Y * Z), y = (i / Z) % Y, z = i % Z;
-    red[i] = scale * full[((2 * x) * NY + 2 * y) * NZ + 2 * z];
-  }
-}
-
-// out[i] = a[i] * b[i]
-static __global__ void mul_kernel(const double* __restrict__ a, const double* __restrict__ b,
-                                  double* __restrict__ out, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    out[i] = a[i] * b[i];
-}
-
-// Host-side expansion / extraction of full-grid fields (the SCFT of a 3m
-// solver with translational parts runs on the expanded fields).
-inline void m3_expand_host(const double* red, double* full, const M3Ops& m, const size_t nred[3]) {
-  const long X = (long)nred[0], Y = (long)nred[1], Z = (long)nred[2];
-  for (long x = 0; x < X; ++x)
-    for (long y = 0; y < Y; ++y)
-      for (long z = 0; z < Z; ++z) {
-        const double v = red[(x * Y + y) * Z + z];
-        for (int o = 0; o < 8; ++o) {
-          const long fx = m3_image(x, m.mask[o] & 1, m.shift[o][0], 2 * X);
-          const long fy = m3_image(y, (m.mask[o] >> 1) & 1, m.shift[o][1], 2 * Y);
-          const long fz = m3_image(z, (m.mask[o] >> 2) & 1, m.shift[o][2], 2 * Z);
-          full[(fx * 2 * Y + fy) * 2 * Z + fz] = v;
-        }
-      }
-}
-
-inline void m3_extract_host(const double* full, double* red, const size_t nred[3], double scale) {
-  const size_t X = nred[0], Y = nred[1], Z = nred[2];
-  for (size_t x = 0; x < X; ++x)
-    for (size_t y = 0; y < Y; ++y)
-      for (size_t z = 0; z < Z; ++z)
-        red[(x * Y + y) * Z + z] = scale * full[((2 * x) * 2 * Y + 2 * y) * 2 * Z + 2 * z];
 }
 
 }  // namespace crys

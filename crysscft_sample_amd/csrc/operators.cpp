@@ -4,8 +4,11 @@
 // a few ulp.
 #include "operators.hpp"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <utility>
 
 namespace crys {
 
@@ -85,6 +88,59 @@ std::vector<double> m3_axis_operator(int N, double L, double ds, bool dL) {
       At[(size_t)i * M + j] = (double)v;
     }
   return At;
+}
+
+std::vector<M3Term> m3_glide_terms(const int mask[8], const int shift[8][3], const int N[3],
+                                   const double L[3], double ds, bool dL) {
+  // a factor is a list of base matrices (mirrored, shift), summed
+  using Base = std::pair<int, int>;
+  using Factor = std::vector<Base>;
+  std::vector<std::array<Factor, 3>> terms;
+  for (int o = 0; o < 8; ++o) {
+    std::array<Factor, 3> t;
+    for (int a = 0; a < 3; ++a) t[a] = {Base{(mask[o] >> a) & 1, shift[o][a]}};
+    terms.push_back(t);
+  }
+  // merge two terms that agree on two axes, adding the third; repeat
+  for (bool merged = true; merged;) {
+    merged = false;
+    for (size_t i = 0; i < terms.size() && !merged; ++i)
+      for (size_t j = i + 1; j < terms.size() && !merged; ++j) {
+        int same = 0, diff = -1;
+        for (int a = 0; a < 3; ++a) {
+          if (terms[i][a] == terms[j][a]) ++same;
+          else diff = a;
+        }
+        if (same < 2) continue;
+        if (diff < 0) diff = 0;  // identical terms: twice one factor
+        Factor f = terms[i][diff];
+        f.insert(f.end(), terms[j][diff].begin(), terms[j][diff].end());
+        std::sort(f.begin(), f.end());
+        terms[i][diff] = f;
+        terms.erase(terms.begin() + (long)j);
+        merged = true;
+      }
+  }
+  std::vector<M3Term> out(terms.size());
+  for (int a = 0; a < 3; ++a) {
+    const int Na = N[a], M = Na / 2;
+    const std::vector<long double> D = p1_dense(Na, L[a], ds, dL);
+    for (size_t t = 0; t < terms.size(); ++t) {
+      std::vector<double>& At = out[t].At[a];
+      At.assign((size_t)M * M, 0.0);
+      for (int j = 0; j < M; ++j)      // output (row of B)
+        for (int i = 0; i < M; ++i) {  // input
+          long double v = 0.0L;
+          for (const Base& b : terms[t][a]) {
+            const long img = b.first ? -1 - 2L * i + b.second : 2L * i + b.second;
+            const long col = ((img % Na) + Na) % Na;
+            v += D[(size_t)(2 * j) * Na + col];
+          }
+          At[(size_t)i * M + j] = (double)v;
+        }
+    }
+  }
+  return out;
 }
 
 std::vector<double> centro_half(const std::vector<double>& At, int M, bool shift) {

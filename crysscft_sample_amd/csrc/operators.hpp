@@ -55,6 +55,24 @@ std::vector<double> centro_half(const std::vector<double>& At, int M, bool shift
 // (k < M, interleaved), then K'_k (k < M): 5 M values.
 std::vector<double> radix_table(int kind, int M, double L, double ds);
 
+// 3m with translational parts (CrysPSSolver3m's translational_part,
+// solver_3m.h:17, folded into its coefficients at solver_3m.h:172-255 with the
+// operations of generate_m3_operation, solver_utilities.h:55-66).  Operation o
+// maps reduced coordinate j of axis a to the full-grid image
+// 2 j + s_oa (not mirrored) or -1 - 2 j + s_oa (mirrored, bit a of mask[o]),
+// mod N_a.  The diffusion on the even sub-lattice is then
+//   out = sum_o B_ox (x) B_oy (x) B_oz,  B_oa[i][j] = Dp1_a[2 i][image_oa(j)],
+// with the P1 operator of each logical axis.  Terms whose factors agree on
+// two axes merge by adding the third; the result is a short list of
+// separable terms (one for zero or mirror-plane-only translations, two for a
+// single glide).  Each factor is returned in the At layout of the passes;
+// with dL the factors are the L derivatives of the same terms (same list).
+struct M3Term {
+  std::vector<double> At[3];
+};
+std::vector<M3Term> m3_glide_terms(const int mask[8], const int shift[8][3], const int N[3],
+                                   const double L[3], double ds, bool dL = false);
+
 // bf16 bits of x, round to nearest even (finite x).
 unsigned short bf16_bits(float x);
 

@@ -79,11 +79,14 @@ CRYS_API int crys_device_synchronize(void);
  * grid_log: logical grid {Nx, Ny, Nz}.  translational_part: 9 doubles for
  * CRYS_3M (the translations of the -z, -y, -x mirrors in cell fractions, as
  * generate_m3_operation, solver_utilities.h:55-66), may be NULL (all zero);
- * ignored for other kinds.  With non-zero parts the solver applies P1's
- * operators on the logical grid to the field expanded by the 8 operations and
- * returns the 2x2y2z sub-lattice; a translation that is not a whole number of
- * grid points, or operations that do not map the sub-lattice one to one onto
- * the grid, give CRYS_ERR_UNSUPPORTED.  precision: enum crys_precision.
+ * ignored for other kinds.  With non-zero parts the solver works on the
+ * 2x2y2z sub-lattice (1/8 of the grid) like the reference's folded
+ * coefficients (solver_3m.h:172-255): its diffusion is a short sum of
+ * separable terms, each the P1 operator of every axis restricted to the
+ * sub-lattice and the images of one symmetry operation; a translation that is
+ * not a whole number of grid points, or operations that do not map the
+ * sub-lattice one to one onto the grid, give CRYS_ERR_UNSUPPORTED.
+ * precision: enum crys_precision.
  * flags: enum crys_flags. */
 CRYS_API int crys_solver_create(int kind, const size_t grid_log[3], const double translational_part[9],
                        int precision, int flags, crys_solver_t** out);

@@ -2,10 +2,13 @@
 
 CrysPSSolver3m takes a translational part g (solver_3m.h:17) that offsets its
 mirrors (glides, shifted mirror planes; generate_m3_operation,
-solver_utilities.h:55-66).  The GPU runs such a solver as expand -> P1 on the
-logical grid -> even sub-lattice; these tests hold it to the oracle's literal
-restatement of the reference's phase-folded 3m solver (itself pinned in
-test_oracle.py by the same identity with the oracle's P1).
+solver_utilities.h:55-66).  The GPU runs such a solver on the 2x2y2z
+sub-lattice itself, as a short sum of separable terms (each the P1 operator
+of every axis restricted to the sub-lattice and the images of one symmetry
+operation); these tests hold it to the oracle's literal restatement of the
+reference's phase-folded 3m solver (itself pinned in test_oracle.py by the
+identity with the oracle's P1 on the expanded field).  The cases reduce to 2,
+8 and 4 separable terms.
 """
 import sys
 
@@ -133,3 +136,43 @@ def test_propagate_matches_oracle(oracle):
     finally:
         hip.hipFree(d)
     assert rel_err(q, qref) < 1e-13
+
+
+@pytest.mark.parametrize("g", [G_CASES[0], G_CASES[2]])
+def test_stress_by_finite_differences(g):
+    """dH/dL of a 3m solver with translational parts (the L derivative of
+    every separable term) against central differences of H."""
+    N, cell, fA, xN, ds = (16, 16, 32), (4.0, 4.1, 4.3), 0.3, 15.0, 1 / 40
+    s = C.Solver3m(N, translational_part=g)
+    run = C.SCFT(s, fA, ds, xN, cell, 0.0)
+    shape = s.get_grid_size_phy()
+    rng = np.random.default_rng(26)
+    wA = 2.0 * rng.standard_normal(shape)
+    wB = -wA + 0.5 * rng.standard_normal(shape)
+    ev = run.evaluate(wA, wB, stress=True)
+    h = 1e-4
+    for a in range(3):
+        def H(d):
+            c = list(cell)
+            c[a] += d
+            s.set_cell_para(c)
+            return run.evaluate(wA, wB, stress=False)["H"]
+        fd = (-H(2 * h) + 8 * H(h) - 8 * H(-h) + H(-2 * h)) / (12 * h)
+        s.set_cell_para(cell)
+        say(f"  dH/dL{a}: fd {fd:.10e} adjoint {ev['dH_dL'][a]:.10e}")
+        assert abs(fd / ev["dH_dL"][a] - 1) < 1e-6
+
+
+def test_fp32_scft_matches_oracle(oracle):
+    N, cell, fA, xN, ds, acc, nstep, g = (32, 32, 32), (4.0,) * 3, 0.3, 20.0, 0.01, 0.1, 3, \
+        G_CASES[0]
+    red = 0.3 * np.random.default_rng(27).standard_normal((16, 16, 16))
+    tr_ref, *_ = oracle.Solver("3m", N, g=g).scft(fA, ds, xN, cell, acc, nstep, xN * red,
+                                                  -xN * red)
+    s = C.Solver3m(N, translational_part=g, precision="fp32")
+    run = C.SCFT(s, fA, ds, xN, cell, acc)
+    run.set_fields(xN * red, -xN * red)
+    tr = run.run(nstep)
+    dF = np.abs(tr[:, 0] / tr_ref[:, 0] - 1).max()
+    say(f"  fp32 3m glide: |dF/F| {dF:.2e}")
+    assert dF < 1e-6
