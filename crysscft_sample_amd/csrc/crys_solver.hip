@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <algorithm>
 #include <array>
 #include <new>
 #include <string>
@@ -450,7 +451,9 @@ int launch_glide_step(const crys_solver* h, const OpSet& op, int nch, const T* c
 // to 64, the radix plane phase when all three axes have length 128 and take
 // the radix pass.
 int phase_mp(const crys_solver* h) {
-  if (!crys::phase_enabled() || h->glide) return 0;
+  if (!crys::phase_enabled()) return 0;
+  if (h->glide)  // the multi-term phases: one length class up to 64 (dense products)
+    return crys::phase_class((int)h->nphy[0], (int)h->nphy[1], (int)h->nphy[2]);
   if (h->nphy[0] == 128 && h->nphy[1] == 128 && h->nphy[2] == 128 && radix_axis(h, 0) &&
       radix_axis(h, 1) && radix_axis(h, 2))
     return 128;
@@ -492,8 +495,10 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
   auto Eb = [&](size_t t) { return t0 + Ns - t - 1 >= NsA ? EB : EA; };
   crys::PhaseChain<T>& f = p.c[0];
   crys::PhaseChain<T>& b = p.c[1];
-  // phase k writes scratch half k % 2 and reads the half phase k-1 wrote
-  const size_t wr = (k % 2) * n, rd = ((k + 1) % 2) * n;
+  // phase k writes scratch half k % 2 and reads the half phase k-1 wrote; a
+  // 3m solver with translational parts keeps one plane per term in each half
+  const size_t nt = h->glide ? op.gmat.size() : 1;
+  const size_t wr = (k % 2) * nt * n, rd = ((k + 1) % 2) * nt * n;
   f.in = k == 0 ? qf : scr[0] + rd;
   b.in = k == 0 ? qb + Ns * n : scr[1] + rd;
   if (k >= 1) {
@@ -507,6 +512,25 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
     f.e_next = Ef(k);
     b.out = scr[1] + wr;
     b.e_next = Eb(k);
+  }
+  if (h->glide) {
+    crys::PhaseTermsParams<T> tp{};
+    tp.c[0] = p.c[0];
+    tp.c[1] = p.c[1];
+    tp.tstride = (long)n;
+    tp.nt = (int)nt;
+    for (size_t t = 0; t < nt && t < (size_t)crys::kMaxGlideTerms; ++t) {
+      tp.mat_r[t] = (const T*)op.gmat[t][ra];
+      tp.mat_c[t] = (const T*)op.gmat[t][2];
+      tp.lo_r[t] = (const unsigned short*)op.glo[t][ra];
+      tp.lo_c[t] = (const unsigned short*)op.glo[t][2];
+    }
+    tp.MR = p.MR;
+    tp.MC = p.MC;
+    tp.plane_stride = p.plane_stride;
+    tp.row_stride = p.row_stride;
+    HIPCHK(crys::launch_phase_terms<T>(tp, nplanes, nch, mp, s));
+    return CRYS_OK;
   }
   if (mp == 128) {
     p.tab_r = (const T*)op.drad[ra];
@@ -1387,9 +1411,11 @@ int crys_scft_create(crys_solver_t* solver, double fA, double ds, double xN, dou
   A(&h->EB, n * es);
   // two fields per chain: the phases alternate between them (a phase may
   // run on two workgroups per plane, so it cannot rewrite its input in place);
-  // three for a 3m solver with translational parts (launch_glide_step: the
-  // term in flight, the partial sum and the next term)
-  const size_t nscr = solver->glide ? 3 : 2;
+  // for a 3m solver with translational parts two per term (its phases keep
+  // one plane per term) and at least three (launch_glide_step: the term in
+  // flight, the partial sum and the next term)
+  const size_t nterms = solver->glide ? solver->ops[solver->cur].gmat.size() : 1;
+  const size_t nscr = solver->glide ? std::max<size_t>(3, 2 * nterms) : 2;
   A(&h->scr[0], nscr * n * es);
   A(&h->scr[1], nscr * n * es);
   A((void**)&h->qpart, crys::kMaxRedBlocks * 8);

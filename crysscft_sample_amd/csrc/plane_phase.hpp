@@ -843,6 +843,188 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   }
 }
 
+// ---------------------------------------------------------------------------
+// One launch per contour step for a 3m solver with translational parts,
+// whose diffusion is a sum of separable terms X_t (x) Y_t (x) Z_t (the folded
+// operator of m3_glide.hpp).  The phases alternate as above, one scratch
+// plane per term: an x-slab phase starts step s with Z_t Y_t (E q_s) for
+// every term t and writes nt planes; the y-slab phase after it finishes the
+// step with sum_t X_t (plane t), accumulated in registers, applies E, writes
+// the history entry, and starts step s + 1 with Z_t X_t (E q_{s+1}) per term;
+// and so on.  3 nt products per phase instead of 3, nt planes read and nt
+// written, one launch per step.  Dense operators (the terms are not
+// centrosymmetric); bufA holds the staged plane for all terms of the start,
+// bufB each term's products.
+constexpr int kMaxGlideTerms = 8;
+
+template <typename T>
+struct PhaseTermsParams {
+  PhaseChain<T> c[2];  // in: term 0's plane (k > 0) or q_0 (start phase); out: term 0's plane
+  long tstride;        // elements between the term planes of in (k > 0) and of out
+  int nt;
+  const T* mat_r[kMaxGlideTerms];  // row-axis factor of each term (X_t or Y_t)
+  const T* mat_c[kMaxGlideTerms];  // z factor of each term
+  const unsigned short* lo_r[kMaxGlideTerms];  // FP32: bf16 low parts, or null
+  const unsigned short* lo_c[kMaxGlideTerms];
+  int MR, MC;
+  long plane_stride, row_stride;
+};
+
+template <typename T, int MP, int NT_>
+__global__ __launch_bounds__(NT_) void plane_phase_terms_kernel(PhaseTermsParams<T> p) {
+  using S = PhaseShape<T, MP, false, NT_>;
+  using V4 = typename MfmaOp<T>::v4;
+  constexpr int NT = S::NT, JT = S::JT, PB = S::PB, PA = S::PA, TPW = S::TPW;
+  constexpr bool SPLIT = std::is_same<T, float>::value;
+  constexpr int KE = (MP * MP + NT - 1) / NT;
+  __shared__ T bufA[MP * PB];
+  __shared__ T bufB[MP * PB];
+  const PhaseChain<T> ch = p.c[blockIdx.y];
+  const long base = (long)blockIdx.x * p.plane_stride, RS = p.row_stride;
+  const int MR = p.MR, MC = p.MC, nt = p.nt;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, jt = w % JT;
+  const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
+
+  long g[KE];
+  bool ok[KE];
+  T ef[KE];
+  const T* e_a = fin ? ch.e_fin : ch.e_next;
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    const int f = tid + NT * k;
+    const int r = f / MP, c = f - r * MP;
+    ok[k] = f < MP * MP && r < MR && c < MC;
+    g[k] = base + r * RS + c;
+  }
+#pragma unroll
+  for (int k = 0; k < KE; ++k) ef[k] = ok[k] ? e_a[g[k]] : T(0);
+  // one plane into bufA (zero outside MR x MC), times E for the start phase
+  auto stage = [&](const T* src, bool times_e) {
+    T x[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) x[k] = ok[k] ? src[g[k]] : T(0);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int r = f / MP, c = f - r * MP;
+      if (f < MP * MP) bufA[r * PB + c] = times_e ? x[k] * ef[k] : x[k];
+    }
+  };
+
+  V4 acc[TPW];
+  if (fin) {
+    // ---- finish step s: sum over the terms of the row factor times its plane ----
+    V4 sum[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) sum[u] = V4{T(0), T(0), T(0), T(0)};
+    for (int t = 0; t < nt; ++t) {
+      stage(ch.in + t * p.tstride, false);
+      __syncthreads();
+      SlabOperands<T, MP, false> orow;
+      slab_load_operands<T, MP, false>(p.mat_r[t], p.lo_r[t], MR, jt, lane, orow);
+      phase_row_product<T, MP, false, NT>(orow, SPLIT && p.lo_r[t] != nullptr, bufA, w, lane,
+                                          acc);
+#pragma unroll
+      for (int u = 0; u < TPW; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sum[u][r] += acc[u][r];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int ct = S::tile(w, u);
+      if (ct >= 0) {
+        const int col = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          lds_put<T>(bufA, (jt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, sum[u][r]);
+      }
+    }
+    __syncthreads();
+    // ---- q_{s+1} = E_s (.) -> history; times E_{s+1} for the next step ----
+    const bool junction = next && ch.e_next != ch.e_fin;
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int r = f / MP, c = f - r * MP;
+      if (f >= MP * MP) continue;
+      T o = T(0);
+      if (ok[k]) {
+        o = bufA[r * PB + c] * ef[k];
+        __builtin_nontemporal_store(o, &ch.hist[g[k]]);
+        if (next) o *= junction ? ch.e_next[g[k]] : ef[k];
+      }
+      if (next) bufA[r * PB + c] = o;
+    }
+    if (!next) return;
+    __syncthreads();
+  } else {
+    stage(ch.in, true);
+    __syncthreads();
+  }
+  // ---- start step s + 1: per term, the row factor then the z factor ----
+  for (int t = 0; t < nt; ++t) {
+    SlabOperands<T, MP, false> orow, ocol;
+    slab_load_operands<T, MP, false>(p.mat_r[t], p.lo_r[t], MR, jt, lane, orow);
+    slab_load_operands<T, MP, false>(p.mat_c[t], p.lo_c[t], MC, jt, lane, ocol);
+    phase_row_product<T, MP, false, NT>(orow, SPLIT && p.lo_r[t] != nullptr, bufA, w, lane,
+                                        acc);
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int ct = S::tile(w, u);
+      if (ct >= 0) {
+        const int col = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          lds_put<T>(bufB, (jt * 16 + MfmaOp<T>::row(lane, r)) * PA + col, acc[u][r]);
+      }
+    }
+    __syncthreads();
+    phase_col_product<T, MP, false, NT>(ocol, SPLIT && p.lo_c[t] != nullptr, bufB, w, lane, acc);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int rt = S::tile(w, u);
+      if (rt >= 0) {
+        const int col = jt * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          lds_put<T>(bufB, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[u][r]);
+      }
+    }
+    __syncthreads();
+    T* out = ch.out + t * p.tstride;
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      const int r = f / MP, c = f - r * MP;
+      if (ok[k]) {
+        if constexpr (SPLIT) out[g[k]] = bufB[r * PB + c];
+        else __builtin_nontemporal_store(bufB[r * PB + c], &out[g[k]]);
+      }
+    }
+    __syncthreads();  // bufB is rewritten by the next term
+  }
+}
+
+// Launches one multi-term phase for 1 or 2 chains (MR, MC <= mp).
+template <typename T>
+hipError_t launch_phase_terms(const PhaseTermsParams<T>& p, int nplanes, int nchains, int mp,
+                              hipStream_t s) {
+  if (nchains < 1 || nchains > 2 || nplanes < 1 || p.nt < 1 || p.nt > kMaxGlideTerms ||
+      p.MR > mp || p.MC > mp)
+    return hipErrorInvalidValue;
+  dim3 grid((unsigned)nplanes, (unsigned)nchains, 1);
+  switch (mp) {
+    case 16: hipLaunchKernelGGL((plane_phase_terms_kernel<T, 16, 1024>), grid, dim3(1024), 0, s, p); break;
+    case 32: hipLaunchKernelGGL((plane_phase_terms_kernel<T, 32, 1024>), grid, dim3(1024), 0, s, p); break;
+    case 48: hipLaunchKernelGGL((plane_phase_terms_kernel<T, 48, 512>), grid, dim3(512), 0, s, p); break;
+    case 64: hipLaunchKernelGGL((plane_phase_terms_kernel<T, 64, 512>), grid, dim3(512), 0, s, p); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 bool phase_half_enabled();
 
 // Whether the phase path applies: all three axes in one length class <= 64.

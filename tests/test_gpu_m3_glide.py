@@ -176,3 +176,52 @@ def test_fp32_scft_matches_oracle(oracle):
     dF = np.abs(tr[:, 0] / tr_ref[:, 0] - 1).max()
     say(f"  fp32 3m glide: |dF/F| {dF:.2e}")
     assert dF < 1e-6
+
+
+GLIDE_CHILD = r"""
+import json, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import crysscft_sample_amd as C
+g, N, prec = json.loads(sys.argv[2])
+N = tuple(N)
+s = C.Solver3m(N, translational_part=g, precision=prec)
+run = C.SCFT(s, 0.3, 0.01, 20.0, (4.0, 4.1, 4.3), 0.1)
+red = 0.3 * np.random.default_rng(28).standard_normal(s.get_grid_size_phy())
+run.set_fields(20.0 * red, -20.0 * red)
+tr = run.run(3)
+_, launches = run.time_propagation(reps=1)
+print(json.dumps({"trace": tr.tolist(), "launches": launches}))
+"""
+
+
+@pytest.mark.parametrize("g", G_CASES)
+@pytest.mark.parametrize("N", [(32, 32, 32), (64, 48, 64)])
+@pytest.mark.parametrize("phase", ["1", "0"])
+def test_scft_paths_match_oracle(oracle, g, N, phase):
+    """The one-launch multi-term phases (default where the reduced axes share a
+    length class up to 64) and the pass-based step (CRYS_PHASE=0) of a 3m
+    solver with translational parts, FP64, against the oracle."""
+    import json
+    import os
+    import subprocess
+
+    from conftest import ROOT
+
+    red_shape = tuple(n // 2 for n in N)
+    red = 0.3 * np.random.default_rng(28).standard_normal(red_shape)
+    tr_ref, *_ = oracle.Solver("3m", N, g=g).scft(0.3, 0.01, 20.0, (4.0, 4.1, 4.3), 0.1, 3,
+                                                  20.0 * red, -20.0 * red)
+    env = dict(os.environ, CRYS_PHASE=phase)
+    r = subprocess.run([sys.executable, "-c", GLIDE_CHILD, ROOT, json.dumps([g, list(N), "fp64"])],
+                       env=env, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    tr = np.array(d["trace"])
+    dF = np.abs(tr[:, 0] / tr_ref[:, 0] - 1).max()
+    dM = np.abs(tr[:, 5] / tr_ref[:, 5] - 1).max()
+    say(f"  3m glide {N} CRYS_PHASE={phase}: {d['launches']} launch(es) per step, "
+        f"|dF/F| {dF:.2e} |dMaxInc| {dM:.2e}")
+    assert dF <= 1e-10 and dM <= 1e-10
+    if phase == "1":
+        assert d["launches"] == 1
