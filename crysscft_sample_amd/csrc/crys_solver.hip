@@ -87,6 +87,17 @@ bool phase_half_enabled() {
   return on;
 }
 
+// CRYS_PHASE_REUSE=0 runs the class-64 split phase with its next row and z
+// products split by k half over all 16 waves (for A/B timing; see
+// plane_phase_half_body).
+bool phase_reuse_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("CRYS_PHASE_REUSE");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
 // CRYS_CENTRO=0 uses the dense operators instead of the centrosymmetric
 // half-size products (for A/B timing).
 bool centro_enabled() {

@@ -624,8 +624,20 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a_khalf(
 // (waves w and w + 8).  The second half's partial tile goes to LDS rows this
 // workgroup does not own (row tile t xor 1, i.e. rows j xor 16) and is added
 // where the tile is read.
-template <typename T>
-__global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p) {
+//
+// The next row product runs on the 8 waves whose finishing row tile is h or
+// 2 + h, over the whole k range with the finishing product's operator
+// registers, and the z product on waves 0..7 over the whole k range: no
+// second row-operator load and no partial tiles to merge.  FP64 also runs the
+// finishing product on waves 0..7 with two column tiles each, sharing the
+// operator registers (FIN8).  Measured per step at 64^3 reduced against the
+// k-half split of the two products over all 16 waves (KHALF, kept for A/B
+// timing): FP64 9.97 -> 9.70 us, FP32 8.22 -> 7.74 us; FIN8 measured 3.9%
+// slower for FP32.
+template <typename T, bool KHALF>
+__device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
+  constexpr bool REUSE = !KHALF, ZFULL = !KHALF;
+  constexpr bool FIN8 = !KHALF && std::is_same<T, double>::value;
   constexpr int MP = 64, H = 32, NT = 1024, JT = 4;
   using S = PhaseShape<T, MP, true, NT>;
   using V4 = typename MfmaOp<T>::v4;
@@ -666,7 +678,7 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   const int r2t = (item & 1) ? 2 + h : h;
   const int ct2 = item >> 1;
   const int rt3 = (item & 1) ? 3 - h : h;
-  SlabOperands<T, MP, true> o1;
+  SlabOperands<T, MP, true> o1, o3f;  // o3f: whole z operator tile (ZFULL)
   KHalfOperands<T, MP> o2, o3;
 
   // ---- global loads up front: the operators, the plane and its E values ----
@@ -681,8 +693,12 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     }
   } else {
     slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, w % JT, lane, o1);
-    load_operands_khalf<T, MP>(p.mat_r, p.lo_r, r2t, lane, kh, o2);
-    load_operands_khalf<T, MP>(p.mat_c, p.lo_c, ct2, lane, kh, o3);
+    if constexpr (!REUSE) load_operands_khalf<T, MP>(p.mat_r, p.lo_r, r2t, lane, kh, o2);
+    if constexpr (ZFULL) {
+      if (w < 8) slab_load_operands<T, MP, true>(p.mat_c, p.lo_c, MP, ct2, lane, o3f);
+    } else {
+      load_operands_khalf<T, MP>(p.mat_c, p.lo_c, ct2, lane, kh, o3);
+    }
   }
   constexpr int KE = H * MP / NT;
   long g1[KE], g2[KE];
@@ -742,11 +758,27 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
   __syncthreads();
 
   if (fin) {
-    if constexpr (OPS_LDS) slab_load_operands<T, MP, true>(opl, nullptr, MP, w % JT, lane, o1, OP);
-    // ---- finishing row product, whole plane, one tile per wave ----
-    V4 acc[TPW];
-    phase_row_product<T, MP, true, NT>(o1, lo_r, buf0, w, lane, acc);
-    {
+    if constexpr (OPS_LDS) {
+      if (!FIN8 || w < 8) slab_load_operands<T, MP, true>(opl, nullptr, MP, w % JT, lane, o1, OP);
+    }
+    // ---- finishing row product, whole plane: one tile per wave, or (FIN8)
+    // two column tiles per wave on waves 0..7 sharing the operator registers ----
+    if constexpr (FIN8) {
+      using S8 = PhaseShape<T, MP, true, 512>;
+      if (w < 8) {
+        V4 acc8[S8::TPW];
+        phase_row_product<T, MP, true, 512>(o1, lo_r, buf0, w, lane, acc8);
+#pragma unroll
+        for (int u = 0; u < S8::TPW; ++u) {
+          const int col = S8::tile(w, u) * 16 + (lane & 15);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            lds_put<T>(buf1, ((w % JT) * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc8[u][r]);
+        }
+      }
+    } else {
+      V4 acc[TPW];
+      phase_row_product<T, MP, true, NT>(o1, lo_r, buf0, w, lane, acc);
       const int ct = S::tile(w, 0);
       const int col = ct * 16 + (lane & 15);
 #pragma unroll
@@ -779,9 +811,41 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     __syncthreads();
   }
 
-  // ---- next row product: U/V row tiles h and 2 + h, k halves ----
-  if constexpr (OPS_LDS) load_operands_khalf<T, MP>(opl, nullptr, r2t, lane, kh, o2, OP);
-  {
+  // ---- next row product: U/V row tiles h and 2 + h ----
+  if constexpr (REUSE && FIN8) {
+    // the waves 0..7 holding row tile h or 2 + h: two column tiles each
+    using S8 = PhaseShape<T, MP, true, 512>;
+    const int rt = w % JT;
+    if (w < 8 && (rt == h || rt == 2 + h)) {
+      if constexpr (OPS_LDS) {
+        if (!fin) slab_load_operands<T, MP, true>(opl, nullptr, MP, rt, lane, o1, OP);
+      }
+      V4 acc8[S8::TPW];
+      phase_row_product<T, MP, true, 512>(o1, lo_r, buf0, w, lane, acc8);
+#pragma unroll
+      for (int u = 0; u < S8::TPW; ++u) {
+        const int col = S8::tile(w, u) * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc8[u][r]);
+      }
+    }
+  } else if constexpr (REUSE) {
+    const int rt = w % JT;
+    if (rt == h || rt == 2 + h) {
+      if constexpr (OPS_LDS) {
+        if (!fin) slab_load_operands<T, MP, true>(opl, nullptr, MP, rt, lane, o1, OP);
+      }
+      V4 acc[TPW];
+      phase_row_product<T, MP, true, NT>(o1, lo_r, buf0, w, lane, acc);
+      const int col = S::tile(w, 0) * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
+    }
+  } else {
+    // k halves over all 16 waves
+    if constexpr (OPS_LDS) load_operands_khalf<T, MP>(opl, nullptr, r2t, lane, kh, o2, OP);
     const V4 acc = mfma_tile_b_khalf<T, MP>(o2, lo_r, buf0, PB, ct2 * 16 + (lane & 15));
     const int rt = kh ? (r2t ^ 1) : r2t;
 #pragma unroll
@@ -796,10 +860,12 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     const int j = 16 * h + tid / H, c = tid - (tid / H) * H, jo = j ^ 16;
     const int cp = cen_partner<MP>(c, shift);
     T nv[4];
-    const T P1 = buf1[j * PB + c] + buf1[jo * PB + c];
-    const T P2 = buf1[(H + j) * PB + c] + buf1[(H + jo) * PB + c];
-    const T P3 = buf1[j * PB + cp] + buf1[jo * PB + cp];
-    const T P4 = buf1[(H + j) * PB + cp] + buf1[(H + jo) * PB + cp];
+    // (the k-half variant sums its two partial tiles, rows j and j ^ 16)
+    const T P1 = REUSE ? buf1[j * PB + c] : buf1[j * PB + c] + buf1[jo * PB + c];
+    const T P2 = REUSE ? buf1[(H + j) * PB + c] : buf1[(H + j) * PB + c] + buf1[(H + jo) * PB + c];
+    const T P3 = REUSE ? buf1[j * PB + cp] : buf1[j * PB + cp] + buf1[jo * PB + cp];
+    const T P4 = REUSE ? buf1[(H + j) * PB + cp]
+                       : buf1[(H + j) * PB + cp] + buf1[(H + jo) * PB + cp];
     const T w11 = P1 + P2, w21 = P1 - P2;
     const T w12 = P3 + P4, w22 = P3 - P4;
     nv[0] = w11 + w12;
@@ -812,9 +878,22 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     buf0[(MP - 1 - j) * PA + H + c] = nv[3];
   }
   __syncthreads();
-  // ---- z product: output row tiles h and 3 - h, k halves ----
-  if constexpr (OPS_LDS) load_operands_khalf<T, MP>(opl + H * OP, nullptr, ct2, lane, kh, o3, OP);
-  {
+  // ---- z product: output row tiles h and 3 - h ----
+  if constexpr (ZFULL) {
+    // 8 waves over the whole k range
+    if (w < 8) {
+      if constexpr (OPS_LDS)
+        slab_load_operands<T, MP, true>(opl + H * OP, nullptr, MP, ct2, lane, o3f, OP);
+      const V4 acc = mfma_tile_a<T, MP, true>(o3f, lo_c, buf0, PA, rt3 * 16 + (lane & 15));
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        lds_put<T>(buf1, (rt3 * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15),
+                   acc[r]);
+    }
+  } else {
+    // k halves over all 16 waves
+    if constexpr (OPS_LDS)
+      load_operands_khalf<T, MP>(opl + H * OP, nullptr, ct2, lane, kh, o3, OP);
     const V4 acc = mfma_tile_a_khalf<T, MP>(o3, lo_c, buf0, PA, rt3 * 16 + (lane & 15));
     const int rt = kh ? (rt3 ^ 1) : rt3;
 #pragma unroll
@@ -829,8 +908,9 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
     const int rr = tid / H, j = tid - (tid / H) * H;
     const int row = rr < 16 ? 16 * h + rr : 48 - 16 * h + (rr - 16);
     const int ro = row ^ 16;
-    const T U = buf1[row * PB + j] + buf1[ro * PB + j];
-    const T V = buf1[row * PB + H + j] + buf1[ro * PB + H + j];
+    // (the k-half variant sums its two partial tiles, rows row and row ^ 16)
+    const T U = ZFULL ? buf1[row * PB + j] : buf1[row * PB + j] + buf1[ro * PB + j];
+    const T V = ZFULL ? buf1[row * PB + H + j] : buf1[row * PB + H + j] + buf1[ro * PB + H + j];
     const long g0 = base + cen_index<MP>(row, shift) * RS;
     const long g3 = g0 + cen_partner<MP>(j, shift);
     if constexpr (SPLIT) {
@@ -841,6 +921,18 @@ __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p
       __builtin_nontemporal_store(U - V, &ch.out[g3]);
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p) {
+  plane_phase_half_body<T, false>(p);
+}
+
+// The same phase with the next row and z products split by k half over all
+// 16 waves (CRYS_PHASE_REUSE=0, for A/B timing).
+template <typename T>
+__global__ __launch_bounds__(1024) void plane_phase_half_khalf_kernel(PhaseParams<T> p) {
+  plane_phase_half_body<T, true>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1026,6 +1118,7 @@ hipError_t launch_phase_terms(const PhaseTermsParams<T>& p, int nplanes, int nch
 }
 
 bool phase_half_enabled();
+bool phase_reuse_enabled();
 
 // Whether the phase path applies: all three axes in one length class <= 64.
 inline int phase_class(int X, int Y, int Z) {
@@ -1054,7 +1147,10 @@ hipError_t launch_phase(const PhaseParams<T>& p, int nplanes, int nchains, int m
     if (mp == 64 && nplanes % 8 == 0 &&
         phase_half_enabled()) {
       dim3 g2((unsigned)(2 * nplanes), (unsigned)nchains, 1);
-      hipLaunchKernelGGL((plane_phase_half_kernel<T>), g2, dim3(1024), 0, s, p);
+      if (phase_reuse_enabled())
+        hipLaunchKernelGGL((plane_phase_half_kernel<T>), g2, dim3(1024), 0, s, p);
+      else
+        hipLaunchKernelGGL((plane_phase_half_khalf_kernel<T>), g2, dim3(1024), 0, s, p);
       return hipGetLastError();
     }
     if (mp == 32)
