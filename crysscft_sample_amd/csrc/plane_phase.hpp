@@ -630,14 +630,18 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a_khalf(
 // registers, and the z product on waves 0..7 over the whole k range: no
 // second row-operator load and no partial tiles to merge.  FP64 also runs the
 // finishing product on waves 0..7 with two column tiles each, sharing the
-// operator registers (FIN8).  Measured per step at 64^3 reduced against the
-// k-half split of the two products over all 16 waves (KHALF, kept for A/B
-// timing): FP64 9.97 -> 9.70 us, FP32 8.22 -> 7.74 us; FIN8 measured 3.9%
-// slower for FP32.
+// operator registers (FIN8).  Wave w runs on SIMD w % 4, so the finishing row
+// tile of wave w is (w + w / 4) % 4 (PERM): the waves that go on to the next
+// row product then fall two (one with FIN8) on each SIMD, where w % 4 would put
+// them all on two SIMDs.  Measured per step at 64^3 reduced against the k-half
+// split of the two products over all 16 waves (KHALF, kept for A/B timing):
+// FP64 9.97 -> 9.70 us (9.17 with PERM), FP32 8.22 -> 7.74 us (7.40 with
+// PERM); FIN8 measured 3-4% slower for FP32, 0.5% faster for FP64.
 template <typename T, bool KHALF>
 __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
   constexpr bool REUSE = !KHALF, ZFULL = !KHALF;
   constexpr bool FIN8 = !KHALF && std::is_same<T, double>::value;
+  constexpr bool PERM = !KHALF;
   constexpr int MP = 64, H = 32, NT = 1024, JT = 4;
   using S = PhaseShape<T, MP, true, NT>;
   using V4 = typename MfmaOp<T>::v4;
@@ -667,6 +671,10 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
   const long RS = p.row_stride;
   const long base = (long)plane * p.plane_stride;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // row tile of wave w's finishing product (and of its next-row tiles): w % 4,
+  // or with PERM (w + w / 4) % 4, which spreads the waves of one row tile over
+  // the four SIMDs (wave w on SIMD w % 4)
+  const int rho = PERM ? ((w + (w >> 2)) & 3) : (w % JT);
   const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
   const bool lo_r = SPLIT && p.lo_r != nullptr, lo_c = SPLIT && p.lo_c != nullptr;
   const bool shift = p.shift != 0;
@@ -692,7 +700,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
       om[1][k] = p.mat_c[tid + NT * k];
     }
   } else {
-    slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, w % JT, lane, o1);
+    slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, rho, lane, o1);
     if constexpr (!REUSE) load_operands_khalf<T, MP>(p.mat_r, p.lo_r, r2t, lane, kh, o2);
     if constexpr (ZFULL) {
       if (w < 8) slab_load_operands<T, MP, true>(p.mat_c, p.lo_c, MP, ct2, lane, o3f);
@@ -759,7 +767,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
 
   if (fin) {
     if constexpr (OPS_LDS) {
-      if (!FIN8 || w < 8) slab_load_operands<T, MP, true>(opl, nullptr, MP, w % JT, lane, o1, OP);
+      if (!FIN8 || w < 8) slab_load_operands<T, MP, true>(opl, nullptr, MP, rho, lane, o1, OP);
     }
     // ---- finishing row product, whole plane: one tile per wave, or (FIN8)
     // two column tiles per wave on waves 0..7 sharing the operator registers ----
@@ -773,7 +781,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
           const int col = S8::tile(w, u) * 16 + (lane & 15);
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            lds_put<T>(buf1, ((w % JT) * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc8[u][r]);
+            lds_put<T>(buf1, (rho * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc8[u][r]);
         }
       }
     } else {
@@ -783,7 +791,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
       const int col = ct * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        lds_put<T>(buf1, ((w % JT) * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
+        lds_put<T>(buf1, (rho * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
     }
     __syncthreads();
     // ---- E, history rows of this half, E of the next step ----
@@ -815,7 +823,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
   if constexpr (REUSE && FIN8) {
     // the waves 0..7 holding row tile h or 2 + h: two column tiles each
     using S8 = PhaseShape<T, MP, true, 512>;
-    const int rt = w % JT;
+    const int rt = rho;
     if (w < 8 && (rt == h || rt == 2 + h)) {
       if constexpr (OPS_LDS) {
         if (!fin) slab_load_operands<T, MP, true>(opl, nullptr, MP, rt, lane, o1, OP);
@@ -831,7 +839,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
       }
     }
   } else if constexpr (REUSE) {
-    const int rt = w % JT;
+    const int rt = rho;
     if (rt == h || rt == 2 + h) {
       if constexpr (OPS_LDS) {
         if (!fin) slab_load_operands<T, MP, true>(opl, nullptr, MP, rt, lane, o1, OP);
