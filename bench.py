@@ -338,10 +338,11 @@ import json, sys, time
 sys.path.insert(0, {root!r})
 import crysscft_sample_amd as C
 from crysscft_sample_amd import fields as F
-name, N, L, prec, iters = json.loads(sys.argv[1])
+name, N, L, T, prec, iters = json.loads(sys.argv[1])
 N = tuple(N)
 wA, wB = F.bcc_fields(N, L, 0.30, 25.0)
-s = C.SOLVERS[name](N, precision=prec)
+s = (C.SOLVERS[name](N, precision=prec) if T is None
+     else C.SOLVERS[name](N, translational_part=T, precision=prec))
 run = C.SCFT(s, 0.30, 1.0 / 200, 25.0, L, 0.1)
 run.set_fields(F.extract(wA, name), F.extract(wB, name))
 run.run(2)
@@ -378,12 +379,17 @@ def passes_timing(iters, only=None, paths=None):
            "unit": "ms", "higher_is_better": False, "grids": {}}
     grids = PASS_GRIDS
     if only:
-        # SOLVER:N or SOLVER:N:L (cell length, default 4.2)
+        # SOLVER:N, SOLVER:N:L (cell length, default 4.2) or SOLVER:N:L:T (3m with
+        # the nine translational parts T, comma-separated)
         grids = []
         for g in only:
             f = g.split(":")
-            grids.append((f[0], int(f[1]), float(f[2]) if len(f) > 2 else 4.2))
-    for name, N, L in grids:
+            T = [float(x) for x in f[3].split(",")] if len(f) > 3 else None
+            if T is not None and len(T) != 9:
+                raise ValueError(f"--pass-grid {g}: T needs 9 values")
+            grids.append((f[0], int(f[1]), float(f[2]) if len(f) > 2 else 4.2, T))
+    for name, N, L, *rest in grids:
+        T = rest[0] if rest else None
         n = (N // 2) ** 3 if name != "P1" else N ** 3
         g = {}
         for prec in ("fp64", "fp32"):
@@ -392,7 +398,7 @@ def passes_timing(iters, only=None, paths=None):
                 if paths and path not in paths:
                     continue
                 env = dict(os.environ, **env_add)
-                arg = json.dumps([name, [N] * 3, L, prec, iters])
+                arg = json.dumps([name, [N] * 3, L, T, prec, iters])
                 print(f"bench: passes {name} {N}^3 {prec} {path}", file=sys.stderr, flush=True)
                 r = subprocess.run([sys.executable, "-c", PASSES_CHILD.format(root=ROOT), arg],
                                    env=env, capture_output=True, text=True, timeout=300)
@@ -403,7 +409,7 @@ def passes_timing(iters, only=None, paths=None):
                 d["step_gbs"] = round(2 * 3 * n * b / (d["ms_step_in_sequence"] * 1e-3) / 1e9, 1)
                 g[f"{prec}/{path}"] = {k: (round(v, 6) if isinstance(v, float) else v)
                                        for k, v in d.items()}
-        out["grids"][f"{name} {N}^3"] = g
+        out["grids"][f"{name} {N}^3" + ("" if T is None else f" T={T}")] = g
     print(json.dumps(out), flush=True)
     return 0
 
@@ -424,8 +430,9 @@ def main():
     ap.add_argument("--passes", action="store_true",
                     help="time the contour step of every kernel path (phase, slab, matrix-core "
                          "passes, radix passes) and exit")
-    ap.add_argument("--pass-grid", action="append", metavar="SOLVER:N[:L]",
-                    help="with --passes: time these grids instead (e.g. PmmmDCT:256, 3m:64:4.0)")
+    ap.add_argument("--pass-grid", action="append", metavar="SOLVER:N[:L[:T]]",
+                    help="with --passes: time these grids instead (e.g. PmmmDCT:256, 3m:64:4.0; "
+                         "T: 3m's nine translational parts, comma-separated)")
     ap.add_argument("--pass-path", action="append", choices=list(PASS_PATHS),
                     help="with --passes: time only these kernel paths")
     ap.add_argument("--sync", action="store_true",
