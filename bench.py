@@ -304,7 +304,8 @@ def batch_timing(count, steps):
     return 0
 
 
-SYNC_VARIANTS = ["launch", "empty_launch", "fenced", "two_level", "bypass", "barrier_only"]
+SYNC_VARIANTS = ["launch", "empty_launch", "fenced", "two_level", "bypass", "barrier_only",
+                 "xcd_local"]
 
 
 def sync_timing(rounds):
@@ -316,7 +317,8 @@ def sync_timing(rounds):
     from crysscft_sample_amd._lib import check, lib
 
     out = {"metric": "us per round: 64 KB per workgroup written, then read by a workgroup on "
-                     "another XCD, synchronised by a launch or a grid barrier",
+                     "another XCD (xcd_local: on the same XCD), synchronised by a launch or a "
+                     "grid barrier",
            "unit": "us", "higher_is_better": False, "rounds": rounds, "workgroups": {}}
     for wg in (64, 128, 256):
         row = {}

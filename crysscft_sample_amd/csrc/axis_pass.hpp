@@ -260,15 +260,18 @@ struct MfmaOp<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-// Writes one accumulator element to LDS.  FP32 writes go through a volatile
-// pointer so that they are never merged into ds_write2_b32 pairs: hipcc
-// (ROCm 7.2, gfx950) rebases such a pair when its offsets exceed the 8-bit
-// dword range and has emitted a wrong offset for one element of the pair
-// (mfma_pass_tile<float, *, 32, true>: byte 240 written instead of 960).
+// Writes one accumulator element to LDS.  FP32 writes are never merged into
+// ds_write2_b32 pairs: hipcc (ROCm 7.2, gfx950) rebases such a pair when its
+// offsets exceed the 8-bit dword range and has emitted a wrong offset for one
+// element of the pair (mfma_pass_tile<float, *, 32, true>: byte 240 written
+// instead of 960).  An empty asm statement with a memory clobber after each
+// store keeps the stores apart.  (A volatile pointer kept them apart too, but
+// compiled to a flat store of system scope followed by a wait for every
+// outstanding global access.)
 template <typename T>
 __device__ __forceinline__ void lds_put(T* p, int idx, T v) {
-  if constexpr (std::is_same<T, float>::value) ((volatile T*)p)[idx] = v;
-  else p[idx] = v;
+  p[idx] = v;
+  if constexpr (std::is_same<T, float>::value) asm volatile("" ::: "memory");
 }
 
 // float -> bf16 bits, round to nearest even (finite inputs).
@@ -278,16 +281,39 @@ __device__ inline short f32_to_bf16(float x) {
   return (short)(u >> 16);
 }
 
-// Far-first order of the KS k groups for a tile whose diagonal sits at group
-// position gc (two pointers from the ends inwards).
-template <int KS>
-__device__ __forceinline__ void far_first_order(float gc, int (&ord)[KS]) {
-  int lo = 0, hi = KS - 1;
+// Far-first order of the KS k groups of an operator tile: the groups sorted
+// by decreasing distance from the tile's diagonal, which sits at group
+// position 4 jb + 1.5 for row block jb (two pointers from the ends inwards),
+// so that the small entries far from the diagonal are summed first.  The
+// orders of all NJB row blocks are built at compile time.
+template <int KS, int NJB>
+struct FarFirstTable {
+  int v[NJB][KS];
+  constexpr FarFirstTable() : v{} {
+    for (int j = 0; j < NJB; ++j) {
+      const float gc = 4.0f * j + 1.5f;
+      int lo = 0, hi = KS - 1;
+      for (int n = 0; n < KS; ++n) {
+        const float dl = lo > gc ? lo - gc : gc - lo;
+        const float dh = hi > gc ? hi - gc : gc - hi;
+        if (dl >= dh) v[j][n] = lo++;
+        else v[j][n] = hi--;
+      }
+    }
+  }
+};
+
+// The order of row block jb (0 <= jb < NJB): selects among the compile-time
+// orders, without branches or a table in memory.
+template <int KS, int NJB>
+__device__ __forceinline__ void far_first_order(int jb, int (&ord)[KS]) {
+  constexpr FarFirstTable<KS, NJB> t{};
 #pragma unroll
   for (int n = 0; n < KS; ++n) {
-    const float dl = fabsf((float)lo - gc), dh = fabsf((float)hi - gc);
-    if (dl >= dh) ord[n] = lo++;
-    else ord[n] = hi--;
+    int o = t.v[0][n];
+#pragma unroll
+    for (int j = 1; j < NJB; ++j) o = jb == j ? t.v[j][n] : o;
+    ord[n] = o;
   }
 }
 
@@ -378,7 +404,7 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
   const int jb = CEN ? (jt % (JT / 2)) : jt;
   const int kbase = CEN ? (jt < JT / 2 ? 0 : H) : 0;
   int ord[KS];
-  far_first_order<KS>(4.0f * jb + 1.5f, ord);
+  far_first_order<KS, CEN ? JT / 2 : JT>(jb, ord);
   const int ldm = CEN ? MP : M;
   T a[KS];
   short alo[SPLIT ? KS : 1];
@@ -626,7 +652,7 @@ __device__ __forceinline__ void slab_load_operands(const T* __restrict__ mat,
   const int jb = CEN ? (jt % half) : jt;
   const int kbase = CEN ? (jt < half ? 0 : MP / 2) : 0;
   int ord[KS];
-  far_first_order<KS>(4.0f * jb + 1.5f, ord);
+  far_first_order<KS, CEN ? JT / 2 : JT>(jb, ord);
   const int jrow = jt * 16 + (lane & 15);
   const int ldm = pitch > 0 ? pitch : (CEN ? MP : M);  // row length of the stored operator
 #pragma unroll

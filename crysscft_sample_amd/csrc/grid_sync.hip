@@ -22,6 +22,12 @@
 //               loaded with agent-scope accesses (write-through, L2 bypass),
 //               so no L2 fence is needed
 //   BARRIER     persistent; FENCED without the data (barrier alone)
+//   XCD         persistent; the workgroups of one XCD (read from the XCC_ID
+//               hardware register) synchronise among themselves on one
+//               counter per XCD, with no L2 fence: the partner is 5 places on
+//               in the same XCD, and its data is read with loads that miss
+//               the CU's L1 and hit the XCD's L2, where every CU's stores
+//               land (the vector L1 is write-through)
 // Every spin is bounded; a workgroup that times out sets a flag that ends
 // every workgroup at its next round, and the count of time-outs is returned.
 #include <hip/hip_runtime.h>
@@ -45,7 +51,8 @@ constexpr size_t kSyncLds = 100 * 1024;  // one workgroup per CU, like the phase
 
 struct SyncArgs {
   double* buf;    // 2 * kSyncMaxWg * kSyncPerWg (double buffer by round parity)
-  unsigned* cnt;  // [0] flat counter; [1..8] per-XCD leaves; [9] root; [16..23] group release
+  unsigned* cnt;  // [0] flat counter; [1..8] per-XCD leaves; [9] root; [16..23] group release;
+                  // [32..39] per-XCD tickets (XCD)
   unsigned* err;  // [0] values not yet visible, [1] spin time-outs
   int nwg, rounds, variant, stride;
 };
@@ -63,26 +70,73 @@ __device__ inline void bounded_spin(unsigned* p, unsigned target, unsigned* err)
 
 __device__ inline double exch_value(int round, int wg) { return (double)(round * 1000 + wg); }
 
+// XCD of the calling wave: the XCC_ID hardware register, bits 3:0.
+__device__ inline int xcd_of_wave() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20); }
+
+// One double stored by another workgroup of the same XCD: the buffer load
+// carries sc0, so it misses this CU's L1 and reads the XCD's shared L2.
+__device__ inline double load_from_l2(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 1));
+}
+
 __global__ __launch_bounds__(kSyncThreads) void sync_persistent_kernel(SyncArgs a) {
   extern __shared__ double sync_lds[];
-  __shared__ int stop;
+  __shared__ int stop, where;
   const int g = blockIdx.x, tid = threadIdx.x;
   sync_lds[tid] = 0.0;
-  if (tid == 0) stop = 0;
   const bool data = a.variant != CRYS_SYNC_BARRIER;
   const bool bypass = a.variant == CRYS_SYNC_BYPASS;
+  const bool xcd = a.variant == CRYS_SYNC_XCD;
+  const int per = a.nwg / 8;
+  // XCD: this workgroup's XCD and its place among the workgroups there
+  if (tid == 0) {
+    stop = 0;
+    where = 0;
+    if (xcd) {
+      const int x = xcd_of_wave();
+      const unsigned t = x < 8 ? __hip_atomic_fetch_add(&a.cnt[32 + x], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                               : 0u;
+      if (x >= 8 || t >= (unsigned)per) {
+        __hip_atomic_fetch_add(&a.err[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        stop = 1;
+      }
+      where = x * per + (int)t;
+    }
+  }
+  __syncthreads();
+  if (stop) return;
+  const int me = xcd ? where : g;
+  const int group = me / per, slot = me - group * per;
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      a.buf, (short)0, (int)(2 * kSyncMaxWg * kSyncPerWg * sizeof(double)), 0x00020000);
   for (int r = 0; r < a.rounds; ++r) {
-    double* mine = a.buf + ((size_t)(r & 1) * a.nwg + g) * kSyncPerWg;
+    const size_t moff = ((size_t)(r & 1) * a.nwg + me) * kSyncPerWg;
+    double* mine = a.buf + moff;
     if (data) {
-      const double v = exch_value(r, g);
+      const double v = exch_value(r, me);
       for (int i = tid; i < kSyncPerWg; i += kSyncThreads) {
-        if (bypass) __hip_atomic_store(&mine[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else __builtin_nontemporal_store(v, &mine[i]);
+        if (bypass) {
+          __hip_atomic_store(&mine[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (xcd) {
+          // an ordinary store (kept in the L2), written as a buffer store so
+          // that it is not merged with the non-temporal store below
+          __builtin_amdgcn_raw_buffer_store_b64(
+              __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), rsrc,
+              (int)((moff + i) * sizeof(double)), 0, 0);
+        } else {
+          __builtin_nontemporal_store(v, &mine[i]);
+        }
       }
     }
+    // every store of this wave has reached the L2 (vmcnt covers stores)
+    if (xcd) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) {
-      if (a.variant == CRYS_SYNC_TWO_LEVEL) {
+      if (xcd) {
+        __hip_atomic_fetch_add(&a.cnt[1 + group], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bounded_spin(&a.cnt[1 + group], (unsigned)(r + 1) * per, a.err);
+      } else if (a.variant == CRYS_SYNC_TWO_LEVEL) {
         const int x = g % 8, per = a.nwg / 8;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned old = __hip_atomic_fetch_add(&a.cnt[1 + x], 1u, __ATOMIC_RELAXED,
@@ -111,14 +165,16 @@ __global__ __launch_bounds__(kSyncThreads) void sync_persistent_kernel(SyncArgs 
     __syncthreads();
     if (stop) return;
     if (data) {
-      const int src = (g + a.stride) % a.nwg;
-      const double* theirs = a.buf + ((size_t)(r & 1) * a.nwg + src) * kSyncPerWg;
+      const int src = xcd ? group * per + (slot + 5) % per : (g + a.stride) % a.nwg;
+      const size_t off = ((size_t)(r & 1) * a.nwg + src) * kSyncPerWg;
+      const double* theirs = a.buf + off;
       const double want = exch_value(r, src);
       unsigned bad = 0;
       for (int i = tid; i < kSyncPerWg; i += kSyncThreads) {
-        const double got = bypass ? __hip_atomic_load(&theirs[i], __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT)
-                                  : theirs[i];
+        const double got =
+            xcd ? load_from_l2(rsrc, (int)((off + i) * sizeof(double)))
+                : bypass ? __hip_atomic_load(&theirs[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : theirs[i];
         bad += got != want;
       }
       if (bad) __hip_atomic_fetch_add(&a.err[0], bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -161,7 +217,7 @@ int sync_fail(int code, const char* what, hipError_t e) {
 
 extern "C" int crys_time_grid_sync(int variant, int workgroups, int rounds, double* us_per_round,
                                    unsigned* not_visible, unsigned* timeouts) {
-  if (variant < CRYS_SYNC_LAUNCH || variant > CRYS_SYNC_BARRIER || rounds < 1 || workgroups < 8 ||
+  if (variant < CRYS_SYNC_LAUNCH || variant > CRYS_SYNC_XCD || rounds < 1 || workgroups < 8 ||
       workgroups > kSyncMaxWg || workgroups % 8 != 0 || !us_per_round)
     return crys::set_error(CRYS_ERR_INVALID_ARGUMENT,
                            "crys_time_grid_sync: variant, 8..256 workgroups (multiple of 8), "

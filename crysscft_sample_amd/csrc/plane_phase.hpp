@@ -545,7 +545,7 @@ __device__ __forceinline__ void load_operands_khalf(const T* __restrict__ mat,
   const int jb = jt % half;
   const int kbase = jt < half ? 0 : MP / 2;
   int ord[KS];
-  far_first_order<KS>(4.0f * jb + 1.5f, ord);
+  far_first_order<KS, JT / 2>(jb, ord);
   const int jrow = jt * 16 + (lane & 15);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -765,6 +765,22 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
   }
   __syncthreads();
 
+  // the history entry of this half (finishing phases), stored at the end of
+  // the phase; non-temporal: the entries are read again only by the density
+  // kernel after the whole propagation
+  const bool junction = fin && next && ch.e_next != ch.e_fin;
+  T hv1[KE], hv2[KE];
+  auto store_history = [&]() {
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int r = (tid + NT * k) / MP;
+      if ((r >> 4) == h) {
+        __builtin_nontemporal_store(hv1[k], &ch.hist[g1[k]]);
+        __builtin_nontemporal_store(hv2[k], &ch.hist[g2[k]]);
+      }
+    }
+  };
+
   if (fin) {
     if constexpr (OPS_LDS) {
       if (!FIN8 || w < 8) slab_load_operands<T, MP, true>(opl, nullptr, MP, rho, lane, o1, OP);
@@ -795,19 +811,18 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
     }
     __syncthreads();
     // ---- E, history rows of this half, E of the next step ----
-    // (history stores are non-temporal: the entries are read again only by
-    // the density kernel after the whole propagation)
-    const bool junction = next && ch.e_next != ch.e_fin;
+    // (the history values stay in registers and are stored at the end of the
+    // phase: a store in flight keeps its data registers, and the compiler
+    // makes the first instruction that reuses them (here the next product's
+    // MFMAs) wait for the store to complete)
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
       const int f = tid + NT * k;
       const int r = f / MP, c = f - r * MP;
       const T U = buf1[r * PB + c], V = buf1[(H + r) * PB + c];
       T o1v = (U + V) * ef1[k], o2v = (U - V) * ef2[k];
-      if ((r >> 4) == h) {
-        __builtin_nontemporal_store(o1v, &ch.hist[g1[k]]);
-        __builtin_nontemporal_store(o2v, &ch.hist[g2[k]]);
-      }
+      hv1[k] = o1v;
+      hv2[k] = o2v;
       if (next) {
         o1v *= junction ? ch.e_next[g1[k]] : ef1[k];
         o2v *= junction ? ch.e_next[g2[k]] : ef2[k];
@@ -815,7 +830,10 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
         buf0[(H + r) * PB + c] = o1v - o2v;
       }
     }
-    if (!next) return;
+    if (!next) {
+      store_history();
+      return;
+    }
     __syncthreads();
   }
 
@@ -929,6 +947,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
       __builtin_nontemporal_store(U - V, &ch.out[g3]);
     }
   }
+  if (fin) store_history();
 }
 
 template <typename T>

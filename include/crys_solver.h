@@ -377,12 +377,18 @@ enum crys_sync_variant {
   CRYS_SYNC_FENCED = 2,    /* persistent kernel, one counter, agent-scope fences */
   CRYS_SYNC_TWO_LEVEL = 3, /* persistent, per-XCD counters then a root, agent-scope fences */
   CRYS_SYNC_BYPASS = 4,    /* persistent, one counter, exchange with agent-scope accesses */
-  CRYS_SYNC_BARRIER = 5    /* persistent, FENCED without the exchange */
+  CRYS_SYNC_BARRIER = 5,   /* persistent, FENCED without the exchange */
+  CRYS_SYNC_XCD = 6        /* persistent, groups of the workgroups on one XCD (read from the
+                              hardware): one counter per XCD, no L2 fence; the partner is
+                              the workgroup 5 places on in the same XCD, read through the
+                              shared L2 with loads that miss the CU's L1 */
 };
 /* Writes the mean microseconds per round, the number of exchanged values a
  * reader found not yet written (must be 0) and the number of bounded spins
- * that timed out (must be 0; a time-out ends every workgroup).  8..256
- * workgroups, a multiple of 8.  Allocates its own buffers. */
+ * that timed out (must be 0; a time-out ends every workgroup; with
+ * CRYS_SYNC_XCD a workgroup beyond workgroups / 8 on one XCD counts as a
+ * time-out too).  8..256 workgroups, a multiple of 8.  Allocates its own
+ * buffers. */
 CRYS_API int crys_time_grid_sync(int variant, int workgroups, int rounds, double* us_per_round,
                                  unsigned* not_visible, unsigned* timeouts);
 

@@ -90,7 +90,7 @@ def test_grid_sync_variants_exchange_correctly(workgroups):
 
     from crysscft_sample_amd._lib import check, lib
 
-    for v in range(6):
+    for v in range(7):
         us = ctypes.c_double()
         bad, tmo = ctypes.c_uint(), ctypes.c_uint()
         check(lib().crys_time_grid_sync(v, workgroups, 50, ctypes.byref(us), ctypes.byref(bad),
