@@ -291,6 +291,18 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
 
   V4 acc[TPW];
   const int jt = w % JT;
+  // the history entry (finishing phases), stored at the end of the phase;
+  // non-temporal: read again only by the density kernel
+  T hv[KE][NV];
+  auto store_history = [&]() {
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      if (f >= TASKS || (!CEN && !ok[k])) continue;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) __builtin_nontemporal_store(hv[k][v], &ch.hist[g[k][v]]);
+    }
+  };
   if (fin) {
     // ---- finish the previous step along the row axis ----
     phase_row_product<T, MP, CEN, NT>(orow, lo_r, lds, w, lane, acc);
@@ -308,7 +320,9 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
     __syncthreads();
 
     // ---- q_{s+1} = E_s (.) -> history; times E_{s+1} for the next step ----
-    // (non-temporal history stores: read again only by the density kernel)
+    // (the history values stay in registers and are stored at the end of the
+    // phase: a store in flight keeps its data registers, and the compiler
+    // makes the first instruction that reuses them wait for the store)
     const bool junction = next && ch.e_next != ch.e_fin;
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
@@ -318,8 +332,8 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
         if (f >= TASKS) continue;
         const T U = lds[r * PB + c], V = lds[(H + r) * PB + c];
         T o1 = (U + V) * ef[k][0], o2 = (U - V) * ef[k][1];
-        __builtin_nontemporal_store(o1, &ch.hist[g[k][0]]);
-        __builtin_nontemporal_store(o2, &ch.hist[g[k][1]]);
+        hv[k][0] = o1;
+        hv[k][1] = o2;
         if (next) {
           o1 *= junction ? ch.e_next[g[k][0]] : ef[k][0];
           o2 *= junction ? ch.e_next[g[k][1]] : ef[k][1];
@@ -330,13 +344,16 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
         T o = T(0);
         if (ok[k]) {
           o = lds[r * PB + c] * ef[k][0];
-          __builtin_nontemporal_store(o, &ch.hist[g[k][0]]);
+          hv[k][0] = o;
           if (next) o *= junction ? ch.e_next[g[k][0]] : ef[k][0];
         }
         if (next) lds[r * PB + c] = o;
       }
     }
-    if (!next) return;
+    if (!next) {
+      store_history();
+      return;
+    }
     __syncthreads();
   }
 
@@ -446,6 +463,7 @@ __device__ __forceinline__ void plane_phase_body(const PhaseChain<T>& ch, const 
       }
     }
   }
+  if (fin) store_history();
 }
 
 template <typename T, int MP, bool CEN, int NT_ = 512>
@@ -1061,6 +1079,9 @@ __global__ __launch_bounds__(NT_) void plane_phase_terms_kernel(PhaseTermsParams
     }
     __syncthreads();
     // ---- q_{s+1} = E_s (.) -> history; times E_{s+1} for the next step ----
+    // (the registers of E_s then hold q_{s+1}, stored after the loop: a load
+    // of the block junction's E between two history stores made each wait
+    // for the stores before it)
     const bool junction = next && ch.e_next != ch.e_fin;
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
@@ -1070,11 +1091,15 @@ __global__ __launch_bounds__(NT_) void plane_phase_terms_kernel(PhaseTermsParams
       T o = T(0);
       if (ok[k]) {
         o = bufA[r * PB + c] * ef[k];
-        __builtin_nontemporal_store(o, &ch.hist[g[k]]);
-        if (next) o *= junction ? ch.e_next[g[k]] : ef[k];
+        const T en = junction ? ch.e_next[g[k]] : ef[k];
+        ef[k] = o;
+        if (next) o *= en;
       }
       if (next) bufA[r * PB + c] = o;
     }
+#pragma unroll
+    for (int k = 0; k < KE; ++k)
+      if (tid + NT * k < MP * MP && ok[k]) __builtin_nontemporal_store(ef[k], &ch.hist[g[k]]);
     if (!next) return;
     __syncthreads();
   } else {

@@ -810,16 +810,34 @@ __global__ __launch_bounds__(512) void radix_phase_kernel(PhaseParams<T> p) {
     slab_transform<T, NT, false, KIND>(pl, tab, tab + 3 * M, p.tab_r + 2 * M, [&] {
       if (!E_EARLY) load_e(efin_b);
     });
-    // q_{s+1} = E_s (.) -> history (non-temporal); times E_{s+1}
+    // q_{s+1} = E_s (.) -> history (non-temporal); times E_{s+1}.  The
+    // registers of E_s then hold q_{s+1}, and the history stores are issued
+    // after the loop: a store in flight keeps its data registers, and a load
+    // of the block junction's E between two stores made each wait for the
+    // stores before it (32 round trips per thread)
     const bool junction = next && ch.e_next != ch.e_fin;
+    if (junction) {
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const int f = tid + NT * k;
+        const int a = lds_at(f);
+        const T q = pl[a] * e[k];
+        pl[a] = q * enext_b[gidx(f)];
+        e[k] = q;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const int a = lds_at(tid + NT * k);
+        const T q = pl[a] * e[k];
+        if (next) pl[a] = q * e[k];
+        e[k] = q;
+      }
+    }
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
       const int f = tid + NT * k;
-      const int a = lds_at(f);
-      const int g = gidx(f);
-      T q = pl[a] * e[k];
-      __builtin_nontemporal_store(q, &hist_b[g]);
-      if (next) pl[a] = q * (junction ? enext_b[g] : e[k]);
+      __builtin_nontemporal_store(e[k], &hist_b[gidx(f)]);
     }
     if (!next) return;
     __syncthreads();
