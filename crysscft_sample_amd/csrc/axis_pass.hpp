@@ -304,15 +304,21 @@ struct FarFirstTable {
 };
 
 // The order of row block jb (0 <= jb < NJB): selects among the compile-time
-// orders, without branches or a table in memory.
-template <int KS, int NJB>
+// orders, without branches or a table in memory.  jb is the same in every
+// lane of the wave (every caller derives it from the wave index).  With
+// SCALAR the selection runs on the scalar unit and the order lives in scalar
+// registers: the FP64 kernels take it (the dense class-64 multi-term phase
+// stopped spilling, 43 -> 32 us per step); FP32 kernels measured faster with
+// the selection in vector registers (the same phase 27.5 against 30.4 us).
+template <int KS, int NJB, bool SCALAR = false>
 __device__ __forceinline__ void far_first_order(int jb, int (&ord)[KS]) {
   constexpr FarFirstTable<KS, NJB> t{};
+  const int j0 = SCALAR ? __builtin_amdgcn_readfirstlane(jb) : jb;
 #pragma unroll
   for (int n = 0; n < KS; ++n) {
     int o = t.v[0][n];
 #pragma unroll
-    for (int j = 1; j < NJB; ++j) o = jb == j ? t.v[j][n] : o;
+    for (int j = 1; j < NJB; ++j) o = j0 == j ? t.v[j][n] : o;
     ord[n] = o;
   }
 }
@@ -404,7 +410,7 @@ __device__ __forceinline__ void mfma_pass_tile(const PassChain<T>& ch, const T* 
   const int jb = CEN ? (jt % (JT / 2)) : jt;
   const int kbase = CEN ? (jt < JT / 2 ? 0 : H) : 0;
   int ord[KS];
-  far_first_order<KS, CEN ? JT / 2 : JT>(jb, ord);
+  far_first_order<KS, CEN ? JT / 2 : JT, sizeof(T) == 8>(jb, ord);
   const int ldm = CEN ? MP : M;
   T a[KS];
   short alo[SPLIT ? KS : 1];
@@ -652,7 +658,7 @@ __device__ __forceinline__ void slab_load_operands(const T* __restrict__ mat,
   const int jb = CEN ? (jt % half) : jt;
   const int kbase = CEN ? (jt < half ? 0 : MP / 2) : 0;
   int ord[KS];
-  far_first_order<KS, CEN ? JT / 2 : JT>(jb, ord);
+  far_first_order<KS, CEN ? JT / 2 : JT, sizeof(T) == 8>(jb, ord);
   const int jrow = jt * 16 + (lane & 15);
   const int ldm = pitch > 0 ? pitch : (CEN ? MP : M);  // row length of the stored operator
 #pragma unroll

@@ -563,7 +563,7 @@ __device__ __forceinline__ void load_operands_khalf(const T* __restrict__ mat,
   const int jb = jt % half;
   const int kbase = jt < half ? 0 : MP / 2;
   int ord[KS];
-  far_first_order<KS, JT / 2>(jb, ord);
+  far_first_order<KS, JT / 2, sizeof(T) == 8>(jb, ord);
   const int jrow = jt * 16 + (lane & 15);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
