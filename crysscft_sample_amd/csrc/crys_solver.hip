@@ -143,6 +143,9 @@ struct OpSet {
   void* dmat_c[3];   // half-size centrosymmetric form (M % 32 == 0; 3m: half-period pairing) or null
   void* dmat_c_lo[3];
   void* dmat_dL[3];  // dD/dL per axis, built on first use (cell stress)
+  // whether the operator of axis a equals the z operator bit for bit (equal
+  // lengths and cell lengths): a phase then loads one operator for both
+  bool same_as_z[3] = {false, false, false};
   void* drad[3];     // radix-pass tables (radix_pass.hpp) when the radix passes are selected
   // 3m with translational parts: the separable terms of the folded operator
   // (crys::m3_glide_terms), per term and axis; their L derivatives on first use
@@ -249,13 +252,16 @@ int upload_matrix(const crys_solver* h, const std::vector<double>& At, void** ds
 // 64 or 128; null elsewhere), paired by the half-period translation for 3m
 // (centro_half).
 int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3] = nullptr,
-                    void* cen[3] = nullptr, void* cen_lo[3] = nullptr) {
-  for (int a = 0; a < 3; ++a) {
-    std::vector<double> At;
+                    void* cen[3] = nullptr, void* cen_lo[3] = nullptr,
+                    bool same_as_z[3] = nullptr) {
+  std::vector<double> ops[3];
+  for (int a = 2; a >= 0; --a) {
+    std::vector<double>& At = ops[a];
     const int M = (int)h->nphy[a];
     if (h->kind == CRYS_PMMM_DCT) At = crys::pmmm_axis_operator(M, h->cell[a], ds, dL);
     else if (h->kind == CRYS_P1) At = crys::p1_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
     else At = crys::m3_axis_operator((int)h->nlog[a], h->cell[a], ds, dL);
+    if (same_as_z) same_as_z[a] = At == ops[2];
     CHK(upload_matrix(h, At, &dst[a], lo ? &lo[a] : nullptr));
     if (cen && M % 32 == 0 && M <= 128 && M != 96 && crys::centro_enabled()) {
       std::vector<double> C = crys::centro_half(At, M, h->kind == CRYS_3M);
@@ -305,7 +311,8 @@ int build_ops(crys_solver* h, double ds, OpSet* out) {
     out->dmat[a] = out->dmat_lo[a] = out->dmat_c[a] = out->dmat_c_lo[a] = out->dmat_dL[a] =
         out->drad[a] = nullptr;
   if (h->glide) return upload_glide_terms(h, ds, false, out);
-  CHK(upload_axis_ops(h, ds, false, out->dmat, out->dmat_lo, out->dmat_c, out->dmat_c_lo));
+  CHK(upload_axis_ops(h, ds, false, out->dmat, out->dmat_lo, out->dmat_c, out->dmat_c_lo,
+                      out->same_as_z));
   for (int a = 0; a < 3; ++a) {
     if (!radix_axis(h, a)) continue;
     const std::vector<double> t = crys::radix_table(radix_kind(h), (int)h->nphy[a], h->cell[a], ds);
@@ -497,6 +504,7 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
   p.mat_c = (const T*)(cen ? op.dmat_c[2] : op.dmat[2]);
   p.lo_r = (const unsigned short*)(cen ? op.dmat_c_lo[ra] : op.dmat_lo[ra]);
   p.lo_c = (const unsigned short*)(cen ? op.dmat_c_lo[2] : op.dmat_lo[2]);
+  p.same_rc = op.same_as_z[ra] ? 1 : 0;
   p.MR = yslab ? X : Y;
   p.MC = Z;
   p.plane_stride = yslab ? (long)Z : (long)Y * Z;
