@@ -142,6 +142,11 @@ struct OpSet {
   void* dmat_lo[3];  // FP32: bf16 bits of D - (float)D (split-precision product)
   void* dmat_c[3];   // half-size centrosymmetric form (M % 32 == 0; 3m: half-period pairing) or null
   void* dmat_c_lo[3];
+  // FP32, half-size operators of length 64: the same values (and bf16 low
+  // parts) in the order of the split phase's operator registers (crys::
+  // load_operands_reg), or null
+  void* dmat_reg[3] = {};
+  void* dmat_reg_lo[3] = {};
   void* dmat_dL[3];  // dD/dL per axis, built on first use (cell stress)
   // whether the operator of axis a equals the z operator bit for bit (equal
   // lengths and cell lengths): a phase then loads one operator for both
@@ -211,6 +216,8 @@ void free_ops(crys_solver* h) {
       if (o.dmat_lo[a]) (void)hipFree(o.dmat_lo[a]);
       if (o.dmat_c[a]) (void)hipFree(o.dmat_c[a]);
       if (o.dmat_c_lo[a]) (void)hipFree(o.dmat_c_lo[a]);
+      if (o.dmat_reg[a]) (void)hipFree(o.dmat_reg[a]);
+      if (o.dmat_reg_lo[a]) (void)hipFree(o.dmat_reg_lo[a]);
       if (o.dmat_dL[a]) (void)hipFree(o.dmat_dL[a]);
       if (o.drad[a]) (void)hipFree(o.drad[a]);
     }
@@ -251,9 +258,36 @@ int upload_matrix(const crys_solver* h, const std::vector<double>& At, void** ds
 // not null it receives the half-size centrosymmetric forms (axes of length 32,
 // 64 or 128; null elsewhere), paired by the half-period translation for 3m
 // (centro_half).
+// The register-order copy of a half-size FP32 operator of length 64 (At_c,
+// 32 x 64): lane l of row tile jt holds, for k step n, the value that
+// slab_load_operands reads for it, At_c[i * 64 + jt * 16 + l % 16] with
+// i = 4 ord[n] + l / 16 (ord: the far-first order of row block jt % 2), at
+// index (jt * 64 + l) * 8 + n; its bf16 low part at the same index.
+int upload_register_order(const std::vector<double>& C, void** dst, void** dst_lo) {
+  constexpr crys::FarFirstTable<8, 2> ff{};
+  std::vector<float> a(4 * 64 * 8);
+  std::vector<unsigned short> l(a.size());
+  for (int jt = 0; jt < 4; ++jt)
+    for (int lane = 0; lane < 64; ++lane)
+      for (int n = 0; n < 8; ++n) {
+        const int i = 4 * ff.v[jt % 2][n] + (lane >> 4);
+        const double v = C[(size_t)i * 64 + jt * 16 + (lane & 15)];
+        const float hi = (float)v;
+        const size_t at = ((size_t)jt * 64 + lane) * 8 + n;
+        a[at] = hi;
+        l[at] = crys::bf16_bits((float)(v - (double)hi));
+      }
+  HIPCHK(hipMalloc(dst, a.size() * sizeof(float)));
+  HIPCHK(hipMemcpy(*dst, a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(dst_lo, l.size() * sizeof(unsigned short)));
+  HIPCHK(hipMemcpy(*dst_lo, l.data(), l.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
+  return CRYS_OK;
+}
+
 int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3] = nullptr,
                     void* cen[3] = nullptr, void* cen_lo[3] = nullptr,
-                    bool same_as_z[3] = nullptr) {
+                    bool same_as_z[3] = nullptr, void* reg[3] = nullptr,
+                    void* reg_lo[3] = nullptr) {
   std::vector<double> ops[3];
   for (int a = 2; a >= 0; --a) {
     std::vector<double>& At = ops[a];
@@ -266,6 +300,8 @@ int upload_axis_ops(crys_solver* h, double ds, bool dL, void* dst[3], void* lo[3
     if (cen && M % 32 == 0 && M <= 128 && M != 96 && crys::centro_enabled()) {
       std::vector<double> C = crys::centro_half(At, M, h->kind == CRYS_3M);
       CHK(upload_matrix(h, C, &cen[a], cen_lo ? &cen_lo[a] : nullptr));
+      if (reg && reg_lo && M == 64 && h->precision == CRYS_FP32)
+        CHK(upload_register_order(C, &reg[a], &reg_lo[a]));
     }
   }
   return CRYS_OK;
@@ -308,11 +344,11 @@ int upload_glide_terms(crys_solver* h, double ds, bool dL, OpSet* o) {
 int build_ops(crys_solver* h, double ds, OpSet* out) {
   out->ds = ds;
   for (int a = 0; a < 3; ++a)
-    out->dmat[a] = out->dmat_lo[a] = out->dmat_c[a] = out->dmat_c_lo[a] = out->dmat_dL[a] =
-        out->drad[a] = nullptr;
+    out->dmat[a] = out->dmat_lo[a] = out->dmat_c[a] = out->dmat_c_lo[a] = out->dmat_reg[a] =
+        out->dmat_reg_lo[a] = out->dmat_dL[a] = out->drad[a] = nullptr;
   if (h->glide) return upload_glide_terms(h, ds, false, out);
   CHK(upload_axis_ops(h, ds, false, out->dmat, out->dmat_lo, out->dmat_c, out->dmat_c_lo,
-                      out->same_as_z));
+                      out->same_as_z, out->dmat_reg, out->dmat_reg_lo));
   for (int a = 0; a < 3; ++a) {
     if (!radix_axis(h, a)) continue;
     const std::vector<double> t = crys::radix_table(radix_kind(h), (int)h->nphy[a], h->cell[a], ds);
@@ -505,6 +541,12 @@ int launch_phase_step(const crys_solver* h, const OpSet& op, size_t k, size_t Ns
   p.lo_r = (const unsigned short*)(cen ? op.dmat_c_lo[ra] : op.dmat_lo[ra]);
   p.lo_c = (const unsigned short*)(cen ? op.dmat_c_lo[2] : op.dmat_lo[2]);
   p.same_rc = op.same_as_z[ra] ? 1 : 0;
+  if (cen && op.dmat_reg[ra] && op.dmat_reg[2]) {
+    p.reg_r = (const T*)op.dmat_reg[ra];
+    p.reg_c = (const T*)op.dmat_reg[2];
+    p.reg_lo_r = (const unsigned short*)op.dmat_reg_lo[ra];
+    p.reg_lo_c = (const unsigned short*)op.dmat_reg_lo[2];
+  }
   p.MR = yslab ? X : Y;
   p.MC = Z;
   p.plane_stride = yslab ? (long)Z : (long)Y * Z;
@@ -844,6 +886,8 @@ int crys_set_contour_step(crys_solver_t* h, double ds) {
       if (o.dmat_lo[a]) (void)hipFree(o.dmat_lo[a]);
       if (o.dmat_c[a]) (void)hipFree(o.dmat_c[a]);
       if (o.dmat_c_lo[a]) (void)hipFree(o.dmat_c_lo[a]);
+      if (o.dmat_reg[a]) (void)hipFree(o.dmat_reg[a]);
+      if (o.dmat_reg_lo[a]) (void)hipFree(o.dmat_reg_lo[a]);
       if (o.drad[a]) (void)hipFree(o.drad[a]);
     }
     for (auto* v : {&o.gmat, &o.glo})

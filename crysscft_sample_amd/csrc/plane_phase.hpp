@@ -59,6 +59,12 @@ struct PhaseParams {
   const unsigned short* lo_c;
   int MR, MC;           // row axis length, z length
   int same_rc;          // the row operator equals the z operator (one copy loaded)
+  // FP32 class-64 half-size operators in register order (load_operands_reg),
+  // or null
+  const T* reg_r;
+  const T* reg_c;
+  const unsigned short* reg_lo_r;
+  const unsigned short* reg_lo_c;
   long plane_stride;    // elements between planes
   long row_stride;      // elements between rows of a plane
 };
@@ -576,6 +582,38 @@ __device__ __forceinline__ void load_operands_khalf(const T* __restrict__ mat,
   }
 }
 
+// Operator registers of row tile jt of a half-size FP32 operator of length 64
+// from its register-order copy (upload_register_order in crys_solver.hip):
+// lane l finds its 8 values, in the far-first order of slab_load_operands, at
+// reg[(jt * 64 + l) * 8 ..] and their bf16 low parts at the same index of
+// lo: three 16-byte loads instead of 16 scattered ones, the same registers.
+__device__ __forceinline__ void load_operands_reg(const float* __restrict__ reg,
+                                                  const unsigned short* __restrict__ lo, int jt,
+                                                  int lane, SlabOperands<float, 64, true>& op) {
+  int ord[8];
+  far_first_order<8, 2>(jt & 1, ord);
+  const int kbase = jt < 2 ? 0 : 32;
+#pragma unroll
+  for (int n = 0; n < 8; ++n) op.brow[n] = kbase + 4 * ord[n] + (lane >> 4);
+  const size_t at = ((size_t)jt * 64 + lane) * 8;
+  const float4 v0 = *(const float4*)(reg + at), v1 = *(const float4*)(reg + at + 4);
+  op.a[0] = v0.x;
+  op.a[1] = v0.y;
+  op.a[2] = v0.z;
+  op.a[3] = v0.w;
+  op.a[4] = v1.x;
+  op.a[5] = v1.y;
+  op.a[6] = v1.z;
+  op.a[7] = v1.w;
+  const uint4 l4 = *(const uint4*)(lo + at);
+  const unsigned lw[4] = {l4.x, l4.y, l4.z, l4.w};
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    op.alo[2 * n] = (short)(lw[n] & 0xffffu);
+    op.alo[2 * n + 1] = (short)(lw[n] >> 16);
+  }
+}
+
 // Partial tile over one k half, the plane as the B operand (see mfma_tile_b).
 template <typename T, int MP>
 __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_b_khalf(
@@ -724,10 +762,23 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
       for (int k = 0; k < KO; ++k) om[1][k] = p.mat_c[tid + NT * k];
     }
   } else {
-    slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, rho, lane, o1);
+    // (register-order copies where they exist: the same values, fewer loads)
+    if constexpr (SPLIT) {
+      if (p.reg_r && p.reg_lo_r) load_operands_reg(p.reg_r, p.reg_lo_r, rho, lane, o1);
+      else slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, rho, lane, o1);
+    } else {
+      slab_load_operands<T, MP, true>(p.mat_r, p.lo_r, MP, rho, lane, o1);
+    }
     if constexpr (!REUSE) load_operands_khalf<T, MP>(p.mat_r, p.lo_r, r2t, lane, kh, o2);
     if constexpr (ZFULL) {
-      if (w < 8) slab_load_operands<T, MP, true>(p.mat_c, p.lo_c, MP, ct2, lane, o3f);
+      if (w < 8) {
+        if constexpr (SPLIT) {
+          if (p.reg_c && p.reg_lo_c) load_operands_reg(p.reg_c, p.reg_lo_c, ct2, lane, o3f);
+          else slab_load_operands<T, MP, true>(p.mat_c, p.lo_c, MP, ct2, lane, o3f);
+        } else {
+          slab_load_operands<T, MP, true>(p.mat_c, p.lo_c, MP, ct2, lane, o3f);
+        }
+      }
     } else {
       load_operands_khalf<T, MP>(p.mat_c, p.lo_c, ct2, lane, kh, o3);
     }
