@@ -741,8 +741,12 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
   // item / 2, k half kh)
   const int item = w & 7, kh = w >> 3;
   const int r2t = (item & 1) ? 2 + h : h;
-  const int ct2 = item >> 1;
-  const int rt3 = (item & 1) ? 3 - h : h;
+  // z product tile of waves 0..7 (ZFULL): the output column tile is the
+  // wave's finishing row tile rho, and the row tile h for waves 0..3, 3 - h
+  // for waves 4..7, so that where the z operator equals the row operator the
+  // finishing product's registers serve the z product too
+  const int ct2 = ZFULL ? rho : item >> 1;
+  const int rt3 = ZFULL ? (w < 4 ? h : 3 - h) : ((item & 1) ? 3 - h : h);
   SlabOperands<T, MP, true> o1, o3f;  // o3f: whole z operator tile (ZFULL)
   KHalfOperands<T, MP> o2, o3;
 
@@ -771,7 +775,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
     }
     if constexpr (!REUSE) load_operands_khalf<T, MP>(p.mat_r, p.lo_r, r2t, lane, kh, o2);
     if constexpr (ZFULL) {
-      if (w < 8) {
+      if (w < 8 && !same_rc) {
         if constexpr (SPLIT) {
           if (p.reg_c && p.reg_lo_c) load_operands_reg(p.reg_c, p.reg_lo_c, ct2, lane, o3f);
           else slab_load_operands<T, MP, true>(p.mat_c, p.lo_c, MP, ct2, lane, o3f);
@@ -983,10 +987,16 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
   // ---- z product: output row tiles h and 3 - h ----
   if constexpr (ZFULL) {
     // 8 waves over the whole k range
+    // (with equal operators o1 serves as the z operator tile: FP32 loads it
+    // at the start, FP64 for the finishing product of waves 0..7)
+    const bool zr = same_rc && (SPLIT || (FIN8 && fin));
     if (w < 8) {
-      if constexpr (OPS_LDS)
-        slab_load_operands<T, MP, true>(opl_z, nullptr, MP, ct2, lane, o3f, OP);
-      const V4 acc = mfma_tile_a<T, MP, true>(o3f, lo_c, buf0, PA, rt3 * 16 + (lane & 15));
+      if constexpr (OPS_LDS) {
+        if (!zr) slab_load_operands<T, MP, true>(opl_z, nullptr, MP, ct2, lane, o3f, OP);
+      }
+      const int arow = rt3 * 16 + (lane & 15);
+      const V4 acc = zr ? mfma_tile_a<T, MP, true>(o1, lo_c, buf0, PA, arow)
+                        : mfma_tile_a<T, MP, true>(o3f, lo_c, buf0, PA, arow);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         lds_put<T>(buf1, (rt3 * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15),
