@@ -156,9 +156,10 @@ def test_fp32_gradient_error_is_bounded():
     a, b = r64.evaluate(wA, wB), r32.evaluate(wA, wB)
     say(f"  fp32: |dH/H| {abs(b['H'] / a['H'] - 1):.2e}, grad {rel_err(b['dH_dwA'], a['dH_dwA']):.2e},"
         f" stress {rel_err(b['dH_dL'], a['dH_dL']):.2e}")
-    assert abs(b["H"] / a["H"] - 1) < 1e-5
-    assert rel_err(b["dH_dwA"], a["dH_dwA"]) < 1e-4
-    assert rel_err(b["dH_dL"], a["dH_dL"]) < 1e-3
+    # measured 8.7e-9, 3.6e-6 and 2.9e-5
+    assert abs(b["H"] / a["H"] - 1) < 5e-8
+    assert rel_err(b["dH_dwA"], a["dH_dwA"]) < 2e-5
+    assert rel_err(b["dH_dL"], a["dH_dL"]) < 2e-4
 
 
 TORCH_SCRIPT = r"""

@@ -91,7 +91,7 @@ def test_random_batch_fp32_error_against_fp64():
         b.close()
     dF = np.abs(out["fp32"][:, :, 0] / out["fp64"][:, :, 0] - 1).max(axis=1)
     say(f"  fp32 batch: per-instance max |dF/F| median {np.median(dF):.2e} max {dF.max():.2e}")
-    assert dF.max() < 1e-5
+    assert dF.max() < 2e-6  # measured 4.6e-7
 
 
 # Every instance with its three reduced axes in one length class: the batch
@@ -136,7 +136,7 @@ def test_phase_batch_matches_oracle_per_instance(oracle, prec, insts):
         if prec == "fp64":
             assert dF <= 1e-10 and dM <= 1e-10
         else:
-            assert dF < 1e-5
+            assert dF < 3e-6  # measured at most 6.9e-7
 
 
 def _phase_batch_run(prec, path, nstep, autotune=False):
@@ -164,7 +164,7 @@ def test_batch_paths_agree(prec):
     assert (used_ph, used_pa) == ("phases", "passes")
     d = rel_err(ph, pa)
     say(f"  {prec} phases vs passes: trace {d:.2e}, phiA {rel_err(phi_ph, phi_pa):.2e}")
-    assert d < (1e-10 if prec == "fp64" else 1e-6)
+    assert d < (1e-10 if prec == "fp64" else 1e-7)  # FP32 measured 2.1e-8
 
 
 def test_batch_autotune_keeps_state():

@@ -137,8 +137,8 @@ def test_fp32_checkpointed_equals_full():
     ck = run_gpu(name, N, cell, a, b, 9, precision="fp32")
     say(f"fp32 checkpointed vs full: trace {rel_err(ck[0], full[0]):.2e} "
         f"phiA {rel_err(ck[1], full[1]):.2e}")
-    assert rel_err(ck[0], full[0]) < 1e-6
-    assert rel_err(ck[1], full[1]) < 1e-6
+    assert rel_err(ck[0], full[0]) < 1e-7  # measured 2.3e-8
+    assert rel_err(ck[1], full[1]) < 5e-7  # measured 9.3e-8
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -275,6 +275,7 @@ def test_fp32_evaluate_checkpointed():
     g0, g1 = res[None], res[12]
     say(f"fp32 evaluate every 12: H {abs(g1['H'] / g0['H'] - 1):.2e} "
         f"grad {rel_err(g1['dH_dwA'], g0['dH_dwA']):.2e} stress {rel_err(g1['dH_dL'], g0['dH_dL']):.2e}")
-    assert abs(g1["H"] / g0["H"] - 1) < 1e-7
-    assert rel_err(g1["dH_dwA"], g0["dH_dwA"]) < 1e-6
-    assert rel_err(g1["dH_dL"], g0["dH_dL"]) < 1e-5
+    # measured 2.6e-9, 7.4e-7 and 6.3e-8
+    assert abs(g1["H"] / g0["H"] - 1) < 2e-8
+    assert rel_err(g1["dH_dwA"], g0["dH_dwA"]) < 3e-6
+    assert rel_err(g1["dH_dL"], g0["dH_dL"]) < 5e-7

@@ -152,8 +152,8 @@ def test_phases_on_non_cubic_grids(oracle, k, prec):
         assert dF <= 1e-10 and dM <= 1e-10
         assert rel_err(gphi, phiA) < 1e-10
     else:
-        assert dF < 1e-6
-        assert np.abs(gphi - phiA).max() < 1e-5
+        assert dF < 1e-6  # measured at most 2.4e-7
+        assert np.abs(gphi - phiA).max() < 3e-6  # measured at most 7.1e-7
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])

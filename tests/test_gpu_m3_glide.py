@@ -175,7 +175,7 @@ def test_fp32_scft_matches_oracle(oracle):
     tr = run.run(nstep)
     dF = np.abs(tr[:, 0] / tr_ref[:, 0] - 1).max()
     say(f"  fp32 3m glide: |dF/F| {dF:.2e}")
-    assert dF < 1e-6
+    assert dF < 3e-7  # measured 4.7e-8
 
 
 GLIDE_CHILD = r"""

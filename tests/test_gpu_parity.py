@@ -223,8 +223,8 @@ def test_scft_fp32_error_is_stated(oracle):
     dF = np.abs(res.trace[:, 0] / tr[:, 0] - 1).max()
     dphi = np.abs(res.phiA - phiA).max()
     say(f"  fp32 propagators: max |dF/F| = {dF:.3e}, max |dphiA| = {dphi:.3e}")
-    assert dF < 1e-6
-    assert dphi < 1e-5
+    assert dF < 3e-7  # measured 6.7e-8
+    assert dphi < 1e-6  # measured 2.3e-7
 
 
 def test_uniform_fields_closed_form_on_gpu():
@@ -291,8 +291,9 @@ def test_scft_one_launch_phases(oracle, name, N, cell, prec):
         assert dF <= 1e-10 and dM <= 1e-10
         assert rel_err(gphiA, phiA) < 1e-10
     else:
-        assert dF < 1e-5
-        assert np.abs(gphiA - phiA).max() < 1e-4
+        # measured at most 1.9e-7 and 1.5e-6 (P1 48^3)
+        assert dF < 1e-6
+        assert np.abs(gphiA - phiA).max() < 5e-6
 
 
 @pytest.mark.parametrize("name", ["PmmmDCT", "3m", "P1"])
@@ -378,7 +379,7 @@ def test_diffusion_class_128(oracle, name, N, cell, ds):
 
 
 # Axis length 128 on all three axes: the one-launch radix plane phase (all
-# three solver kinds), FP64 against the oracle at 1e-10 and FP32 at 1e-5.
+# three solver kinds), FP64 against the oracle at 1e-10 and FP32 at 1e-8 in F.
 SCFT_128 = [("PmmmDCT", (256,) * 3, (5.0,) * 3, "fp64"), ("P1", (128,) * 3, (4.4,) * 3, "fp64"),
             ("3m", (256,) * 3, (5.0,) * 3, "fp64"), ("PmmmDCT", (256,) * 3, (5.0,) * 3, "fp32")]
 
@@ -398,9 +399,10 @@ def test_scft_class_128(oracle, name, N, cell, prec):
     dF = np.abs(tr[:, 0] / tr_ref[:, 0] - 1).max()
     dM = np.abs(tr[:, 5] / tr_ref[:, 5] - 1).max()
     say(f"  {name} {N} {prec}: |dF/F| {dF:.2e} |dMaxInc| {dM:.2e}")
-    tol = 1e-10 if prec == "fp64" else 1e-5
-    assert dF <= tol and dM <= (tol if prec == "fp64" else 1e-3)
-    assert rel_err(run.fields()[0], phiA_ref) < (1e-10 if prec == "fp64" else 1e-5)
+    # FP32 measured |dF/F| 1.3e-9, |dMaxInc| 2.1e-7
+    tol = 1e-10 if prec == "fp64" else 1e-8
+    assert dF <= tol and dM <= (tol if prec == "fp64" else 1e-6)
+    assert rel_err(run.fields()[0], phiA_ref) < (1e-10 if prec == "fp64" else 2e-6)
     run.close()
     s.close()
 
