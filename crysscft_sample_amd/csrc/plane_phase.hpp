@@ -694,7 +694,7 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a_khalf(
 // split of the two products over all 16 waves (KHALF, kept for A/B timing):
 // FP64 9.97 -> 9.70 us (9.17 with PERM), FP32 8.22 -> 7.74 us (7.40 with
 // PERM); FIN8 measured 3-4% slower for FP32, 0.5% faster for FP64.
-template <typename T, bool KHALF>
+template <typename T, bool KHALF, bool FIN>
 __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
   constexpr bool REUSE = !KHALF, ZFULL = !KHALF;
   constexpr bool FIN8 = !KHALF && std::is_same<T, double>::value;
@@ -732,7 +732,10 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
   // or with PERM (w + w / 4) % 4, which spreads the waves of one row tile over
   // the four SIMDs (wave w on SIMD w % 4)
   const int rho = PERM ? ((w + (w >> 2)) & 3) : (w % JT);
-  const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
+  // FIN: the phase finishes a step (every phase but the start phase); both
+  // chains are at the same phase
+  constexpr bool fin = FIN;
+  const bool next = ch.out != nullptr;
   const bool lo_r = SPLIT && p.lo_r != nullptr, lo_c = SPLIT && p.lo_c != nullptr;
   const bool shift = p.shift != 0;
 
@@ -797,22 +800,25 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
     g1[k] = base + r * RS + c;
     g2[k] = base + cen_partner<MP>(r, shift) * RS + c;
   }
-  // the plane first, then one E (the finished step's, or the next step's in
-  // a start phase): unconditional loads in a fixed order, so that staging
-  // waits only for the plane; the other E of a block junction is read later.
   // The plane is read once, non-temporally, so that it does not displace the
   // E values (read again by every step) from this XCD's L2: measured 0.7% to
   // 2.2% faster per FP64 step at 64^3 reduced (two machines), FP32 within 0.5%.
-  const T* e_a = fin ? ch.e_fin : ch.e_next;
+  // A start phase stages the plane times E of the next step and reads that E
+  // behind the plane; a finishing phase reads its E after the staging
+  // barrier, while the finishing product runs, so that the plane's loads do
+  // not share the memory system with it.  The other E of a block junction is
+  // read in the E/history stage.
 #pragma unroll
   for (int k = 0; k < KE; ++k) {
     x1[k] = __builtin_nontemporal_load(&ch.in[g1[k]]);
     x2[k] = __builtin_nontemporal_load(&ch.in[g2[k]]);
   }
+  if constexpr (!FIN) {
 #pragma unroll
-  for (int k = 0; k < KE; ++k) {
-    ef1[k] = e_a[g1[k]];
-    ef2[k] = e_a[g2[k]];
+    for (int k = 0; k < KE; ++k) {
+      ef1[k] = ch.e_next[g1[k]];
+      ef2[k] = ch.e_next[g2[k]];
+    }
   }
 
   // ---- operators to LDS; stage u/v rows (start: times E of the next step) ----
@@ -844,6 +850,13 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
     }
   }
   __syncthreads();
+  if constexpr (FIN) {
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      ef1[k] = ch.e_fin[g1[k]];
+      ef2[k] = ch.e_fin[g2[k]];
+    }
+  }
 
   // the history entry of this half (finishing phases), stored at the end of
   // the phase; non-temporal: the entries are read again only by the density
@@ -1035,16 +1048,16 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
   if (fin) store_history();
 }
 
-template <typename T>
+template <typename T, bool FIN>
 __global__ __launch_bounds__(1024) void plane_phase_half_kernel(PhaseParams<T> p) {
-  plane_phase_half_body<T, false>(p);
+  plane_phase_half_body<T, false, FIN>(p);
 }
 
 // The same phase with the next row and z products split by k half over all
 // 16 waves (CRYS_PHASE_REUSE=0, for A/B timing).
-template <typename T>
+template <typename T, bool FIN>
 __global__ __launch_bounds__(1024) void plane_phase_half_khalf_kernel(PhaseParams<T> p) {
-  plane_phase_half_body<T, true>(p);
+  plane_phase_half_body<T, true, FIN>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1266,10 +1279,18 @@ hipError_t launch_phase(const PhaseParams<T>& p, int nplanes, int nchains, int m
     if (mp == 64 && nplanes % 8 == 0 &&
         phase_half_enabled()) {
       dim3 g2((unsigned)(2 * nplanes), (unsigned)nchains, 1);
-      if (phase_reuse_enabled())
-        hipLaunchKernelGGL((plane_phase_half_kernel<T>), g2, dim3(1024), 0, s, p);
-      else
-        hipLaunchKernelGGL((plane_phase_half_khalf_kernel<T>), g2, dim3(1024), 0, s, p);
+      // (both chains finish a step in every phase but the start phase)
+      const bool fin = p.c[0].hist != nullptr;
+      if (nchains == 2 && (p.c[1].hist != nullptr) != fin) return hipErrorInvalidValue;
+      if (phase_reuse_enabled()) {
+        if (fin) hipLaunchKernelGGL((plane_phase_half_kernel<T, true>), g2, dim3(1024), 0, s, p);
+        else hipLaunchKernelGGL((plane_phase_half_kernel<T, false>), g2, dim3(1024), 0, s, p);
+      } else {
+        if (fin)
+          hipLaunchKernelGGL((plane_phase_half_khalf_kernel<T, true>), g2, dim3(1024), 0, s, p);
+        else
+          hipLaunchKernelGGL((plane_phase_half_khalf_kernel<T, false>), g2, dim3(1024), 0, s, p);
+      }
       return hipGetLastError();
     }
     if (mp == 32)
