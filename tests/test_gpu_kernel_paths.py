@@ -6,8 +6,9 @@ plane phase when every axis has length 128), the fused z+y slab (matrix-core,
 or radix at length 128) plus the x pass, or three axis passes; the
 centrosymmetric products where they apply (mirror pairing for PmmmDCT and
 P1, half-period pairing for 3m).  The environment switches
-CRYS_PHASE, CRYS_PHASE_HALF, CRYS_SLAB, CRYS_CENTRO and CRYS_AXIS_PASS force
-the other paths; each is read once per process, so each case runs the GPU
+CRYS_PHASE, CRYS_PHASE_HALF, CRYS_PHASE_REUSE (the split phase with its
+next-row and z products split by k half), CRYS_SLAB, CRYS_CENTRO and
+CRYS_AXIS_PASS force the other paths; each is read once per process, so each case runs the GPU
 side in a child process.  FP64 traces must stay within 1e-10 of the oracle on
 every path.
 """
@@ -46,7 +47,7 @@ print(json.dumps(run.run(nstep).tolist()))
 CASES = [("PmmmDCT", (128, 128, 128), (4.2,) * 3, 0.01), ("P1", (48, 48, 48), (4.2,) * 3, 0.01),
          ("PmmmDCT", (256, 256, 256), (5.0,) * 3, 0.05), ("3m", (128, 128, 128), (4.2,) * 3, 0.01),
          ("3m", (64, 64, 64), (4.0,) * 3, 0.01)]
-SWITCHES = [{}, {"CRYS_PHASE_HALF": "0"}, {"CRYS_PHASE": "0"},
+SWITCHES = [{}, {"CRYS_PHASE_HALF": "0"}, {"CRYS_PHASE_REUSE": "0"}, {"CRYS_PHASE": "0"},
             {"CRYS_PHASE": "0", "CRYS_SLAB": "0"}, {"CRYS_CENTRO": "0"},
             {"CRYS_AXIS_PASS": "mfma"}]
 FA, XN, ACC, NSTEP = 0.3, 25.0, 0.1, 2

@@ -296,6 +296,50 @@ def test_scft_one_launch_phases(oracle, name, N, cell, prec):
         assert np.abs(gphiA - phiA).max() < 5e-6
 
 
+# PmmmDCT 128^3 (64^3 reduced) on an orthorhombic cell: the two-workgroup
+# class-64 phase with three different axis operators (the row operator of
+# every phase differs from the z operator).  The oracle's Pmmm solver keeps
+# the reference's cubic cell, so the reference is its P1 solver on the full
+# mirror-symmetric fields; ds = 0.025 keeps that under half a minute.
+ORTHO = dict(N=(128, 128, 128), cell=(4.0, 4.1, 4.3), fA=0.3, xN=25.0, ds=0.025, acc=0.1,
+             nstep=2)
+
+
+@pytest.fixture(scope="module")
+def ortho_ref(oracle):
+    c = ORTHO
+    wA, wB = F.bcc_fields(c["N"], c["cell"], c["fA"], c["xN"])
+    tr, _, _, phiA, _ = oracle.Solver("P1", c["N"]).scft(c["fA"], c["ds"], c["xN"], c["cell"],
+                                                         c["acc"], c["nstep"], wA, wB)
+    return F.extract(wA, "PmmmDCT"), F.extract(wB, "PmmmDCT"), tr, F.extract(phiA, "PmmmDCT")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_class64_phase_with_distinct_axis_operators(ortho_ref, prec):
+    c = ORTHO
+    a, b, tr, phiA = ortho_ref
+    say("class-64 phase, orthorhombic cell", c["cell"], prec)
+    s = C.SolverPmmmDCT(c["N"], orthorhombic=True, precision=prec)
+    run = C.SCFT(s, c["fA"], c["ds"], c["xN"], c["cell"], c["acc"])
+    run.set_fields(a, b)
+    got = run.run(c["nstep"])
+    _, launches = run.time_propagation(reps=1)
+    gphiA = run.fields()[0]
+    dF = np.abs(got[:, 0] / tr[:, 0] - 1).max()
+    dM = np.abs(got[:, 5] / tr[:, 5] - 1).max()
+    say(f"  {launches} launch(es) per step, |dF/F| {dF:.2e} |dMaxInc| {dM:.2e} "
+        f"|dphiA| {np.abs(gphiA - phiA).max():.2e}")
+    assert launches == 1
+    if prec == "fp64":
+        assert dF <= 1e-10 and dM <= 1e-10
+        assert rel_err(gphiA, phiA) < 1e-10
+    else:
+        assert dF < 1e-6
+        assert np.abs(gphiA - phiA).max() < 5e-6
+    run.close()
+    s.close()
+
+
 @pytest.mark.parametrize("name", ["PmmmDCT", "3m", "P1"])
 def test_scft_200_iterations_against_golden(golden_scft, name):
     """The GPU workload in full: 64^3 grid, Ns = 200, 200 mixing iterations,
