@@ -44,7 +44,7 @@ MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # one-launch plane phase (all axes in one length class <= 64), the fused z+y
 # slab plus the x pass, or three axis passes.
 STEP_KERNELS = {
-    1: ["crys::plane_phase_half_kernel<double>" if os.environ.get("CRYS_PHASE_HALF") != "0"
+    1: ["crys::plane_phase_half_kernel<double, true>" if os.environ.get("CRYS_PHASE_HALF") != "0"
         else "crys::plane_phase_kernel<double,64,true,1024>"],
     2: ["crys::slab_zy_kernel<double,64,true,true>",
         "crys::axis_pass_mfma_kernel<double,0,64,true,true>"],
