@@ -780,9 +780,12 @@ __global__ __launch_bounds__(512) void radix_phase_kernel(PhaseParams<T> p) {
   for (int f = tid; f < 4 * M; f += NT)
     tab[f] = f < 2 * M ? p.tab_c[f] : (f < 3 * M ? p.tab_c[f + 2 * M] : p.tab_r[f + M]);
   // the plane and E (the finished step's; the next step's in a start phase),
-  // both in the coalesced element order f = tid + NT k.  FP64 loads the
-  // finished step's E only before the last stage of the finishing transform:
-  // held through the whole transform its 64 registers would spill.
+  // both in the coalesced element order f = tid + NT k.  A start phase reads
+  // E behind the plane (its staging multiplies by it).  A finishing phase
+  // reads the finished step's E after the staging barrier, so that the plane
+  // does not share the memory system with it; FP64 only before the last
+  // stage of the finishing transform: held through the whole transform its
+  // 64 registers would spill.
   constexpr bool E_EARLY = sizeof(T) == 4;
   T e[KT];
   auto load_e = [&](const T* ea) {
@@ -799,11 +802,18 @@ __global__ __launch_bounds__(512) void radix_phase_kernel(PhaseParams<T> p) {
       else v[k] = in_b[gidx(tid + NT * k)];
     }
     if (!fin) load_e(enext_b);
-    else if (E_EARLY) load_e(efin_b);
+    // (FP32: two loops, so that a finishing phase's staging does not wait
+    // for E)
+    if (fin && E_EARLY) {
 #pragma unroll
-    for (int k = 0; k < KT; ++k) pl[lds_at(tid + NT * k)] = fin ? v[k] : v[k] * e[k];
+      for (int k = 0; k < KT; ++k) pl[lds_at(tid + NT * k)] = v[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < KT; ++k) pl[lds_at(tid + NT * k)] = fin ? v[k] : v[k] * e[k];
+    }
   }
   __syncthreads();
+  if (fin && E_EARLY) load_e(efin_b);
 
   if (fin) {
     // finish the previous step along the row axis
