@@ -916,10 +916,13 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
     }
     __syncthreads();
     // ---- E, history rows of this half, E of the next step ----
-    // (the history values stay in registers and are stored at the end of the
-    // phase: a store in flight keeps its data registers, and the compiler
-    // makes the first instruction that reuses them (here the next product's
-    // MFMAs) wait for the store to complete)
+    // (FP32 issues the history stores here, and they complete while the next
+    // products run: 1% faster per step than at the end of the phase.  A store
+    // in flight keeps its data registers, and the compiler makes the first
+    // instruction that reuses them wait for the store to complete, so the
+    // history values are held to the end of the phase, where nothing has
+    // reused their registers.  FP64 stores them at the end of the phase,
+    // measured 1% faster than here)
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
       const int f = tid + NT * k;
@@ -935,10 +938,8 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
         buf0[(H + r) * PB + c] = o1v - o2v;
       }
     }
-    if (!next) {
-      store_history();
-      return;
-    }
+    if (SPLIT || !next) store_history();
+    if (!next) return;
     __syncthreads();
   }
 
@@ -1057,7 +1058,15 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
       __builtin_nontemporal_store(U - V, &ch.out[g3]);
     }
   }
-  if (fin) store_history();
+  if constexpr (FIN) {
+    if constexpr (SPLIT) {
+      // (the history values of the stores in flight)
+#pragma unroll
+      for (int k = 0; k < KE; ++k) asm volatile("" ::"v"(hv1[k]), "v"(hv2[k]));
+    } else {
+      store_history();
+    }
+  }
 }
 
 template <typename T, bool FIN>
