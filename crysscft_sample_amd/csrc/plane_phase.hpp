@@ -1108,17 +1108,26 @@ struct PhaseTermsParams {
   long plane_stride, row_stride;
 };
 
-template <typename T, int MP, int NT_>
+// HALF: two workgroups per plane (workgroup ids 8 apart, which places them on
+// one XCD), as the class-64 split phase: each finishes the step for the whole
+// plane (the next row product needs every row) and starts the next one for
+// its half h of the rows only, one output tile per wave: 2 nt products per
+// workgroup instead of 3 nt, on twice the workgroups.
+template <typename T, int MP, int NT_, bool HALF = false>
 __global__ __launch_bounds__(NT_) void plane_phase_terms_kernel(PhaseTermsParams<T> p) {
   using S = PhaseShape<T, MP, false, NT_>;
   using V4 = typename MfmaOp<T>::v4;
   constexpr int NT = S::NT, JT = S::JT, PB = S::PB, PA = S::PA, TPW = S::TPW;
   constexpr bool SPLIT = std::is_same<T, float>::value;
   constexpr int KE = (MP * MP + NT - 1) / NT;
+  static_assert(!HALF || NT / 64 == (JT / 2) * JT, "one start tile per wave");
   __shared__ T bufA[MP * PB];
   __shared__ T bufB[MP * PB];
   const PhaseChain<T> ch = p.c[blockIdx.y];
-  const long base = (long)blockIdx.x * p.plane_stride, RS = p.row_stride;
+  const int id = blockIdx.x;
+  const int h = HALF ? (id >> 3) & 1 : 0;
+  const int plane = HALF ? (((id >> 4) << 3) | (id & 7)) : id;
+  const long base = (long)plane * p.plane_stride, RS = p.row_stride;
   const int MR = p.MR, MC = p.MC, nt = p.nt;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, jt = w % JT;
   const bool fin = ch.hist != nullptr, next = ch.out != nullptr;
@@ -1199,13 +1208,54 @@ __global__ __launch_bounds__(NT_) void plane_phase_terms_kernel(PhaseTermsParams
       if (next) bufA[r * PB + c] = o;
     }
 #pragma unroll
-    for (int k = 0; k < KE; ++k)
-      if (tid + NT * k < MP * MP && ok[k]) __builtin_nontemporal_store(ef[k], &ch.hist[g[k]]);
+    for (int k = 0; k < KE; ++k) {
+      const int f = tid + NT * k;
+      if (f < MP * MP && ok[k] && (!HALF || (f / MP >= MP / 2) == (h != 0)))
+        __builtin_nontemporal_store(ef[k], &ch.hist[g[k]]);
+    }
     if (!next) return;
     __syncthreads();
   } else {
     stage(ch.in, true);
     __syncthreads();
+  }
+  if constexpr (HALF) {
+    // ---- start step s + 1 for rows [MP/2 h, MP/2 (h + 1)): per term, wave w
+    // computes row tile rt and column tile ct of the row factor, then of the
+    // z factor ----
+    const int rt = (JT / 2) * h + w % (JT / 2), ct = w / (JT / 2);
+    constexpr int KH = (MP / 2 * MP + NT - 1) / NT;
+    for (int t = 0; t < nt; ++t) {
+      SlabOperands<T, MP, false> orow, ocol;
+      slab_load_operands<T, MP, false>(p.mat_r[t], p.lo_r[t], MR, rt, lane, orow);
+      slab_load_operands<T, MP, false>(p.mat_c[t], p.lo_c[t], MC, ct, lane, ocol);
+      const V4 a = mfma_tile_b<T, MP, false>(orow, SPLIT && p.lo_r[t] != nullptr, bufA, PB,
+                                             ct * 16 + (lane & 15));
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        lds_put<T>(bufB, (rt * 16 + MfmaOp<T>::row(lane, r)) * PA + ct * 16 + (lane & 15), a[r]);
+      __syncthreads();
+      const V4 z = mfma_tile_a<T, MP, false>(ocol, SPLIT && p.lo_c[t] != nullptr, bufB, PA,
+                                             rt * 16 + (lane & 15));
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        lds_put<T>(bufB, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct * 16 + (lane & 15), z[r]);
+      __syncthreads();
+      T* out = ch.out + t * p.tstride;
+#pragma unroll
+      for (int k = 0; k < KH; ++k) {
+        const int f = tid + NT * k;
+        const int r = (MP / 2) * h + f / MP, c = f % MP;
+        if (f < MP / 2 * MP && r < MR && c < MC) {
+          const long gi = base + r * RS + c;
+          if constexpr (SPLIT) out[gi] = bufB[r * PB + c];
+          else __builtin_nontemporal_store(bufB[r * PB + c], &out[gi]);
+        }
+      }
+      __syncthreads();  // bufB is rewritten by the next term
+    }
+    return;
   }
   // ---- start step s + 1: per term, the row factor then the z factor ----
   for (int t = 0; t < nt; ++t) {
@@ -1252,6 +1302,8 @@ __global__ __launch_bounds__(NT_) void plane_phase_terms_kernel(PhaseTermsParams
   }
 }
 
+bool phase_half_enabled();
+
 // Launches one multi-term phase for 1 or 2 chains (MR, MC <= mp).
 template <typename T>
 hipError_t launch_phase_terms(const PhaseTermsParams<T>& p, int nplanes, int nchains, int mp,
@@ -1264,13 +1316,20 @@ hipError_t launch_phase_terms(const PhaseTermsParams<T>& p, int nplanes, int nch
     case 16: hipLaunchKernelGGL((plane_phase_terms_kernel<T, 16, 1024>), grid, dim3(1024), 0, s, p); break;
     case 32: hipLaunchKernelGGL((plane_phase_terms_kernel<T, 32, 1024>), grid, dim3(1024), 0, s, p); break;
     case 48: hipLaunchKernelGGL((plane_phase_terms_kernel<T, 48, 512>), grid, dim3(512), 0, s, p); break;
-    case 64: hipLaunchKernelGGL((plane_phase_terms_kernel<T, 64, 512>), grid, dim3(512), 0, s, p); break;
+    case 64:
+      // two workgroups per plane where the planes pair up on the XCDs
+      if (nplanes % 8 == 0 && phase_half_enabled()) {
+        dim3 g2((unsigned)(2 * nplanes), (unsigned)nchains, 1);
+        hipLaunchKernelGGL((plane_phase_terms_kernel<T, 64, 512, true>), g2, dim3(512), 0, s, p);
+      } else {
+        hipLaunchKernelGGL((plane_phase_terms_kernel<T, 64, 512>), grid, dim3(512), 0, s, p);
+      }
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-bool phase_half_enabled();
 bool phase_reuse_enabled();
 
 // Whether the phase path applies: all three axes in one length class <= 64.
