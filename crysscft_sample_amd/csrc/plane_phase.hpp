@@ -1161,20 +1161,44 @@ __global__ __launch_bounds__(NT_) void plane_phase_terms_kernel(PhaseTermsParams
   V4 acc[TPW];
   if (fin) {
     // ---- finish step s: sum over the terms of the row factor times its plane ----
+    // (two terms' planes are read at once, into bufA and bufB, so that the
+    // second one's memory latency overlaps the first one's)
     V4 sum[TPW];
 #pragma unroll
     for (int u = 0; u < TPW; ++u) sum[u] = V4{T(0), T(0), T(0), T(0)};
-    for (int t = 0; t < nt; ++t) {
-      stage(ch.in + t * p.tstride, false);
-      __syncthreads();
+    auto add_term = [&](int t, const T* buf) {
       SlabOperands<T, MP, false> orow;
       slab_load_operands<T, MP, false>(p.mat_r[t], p.lo_r[t], MR, jt, lane, orow);
-      phase_row_product<T, MP, false, NT>(orow, SPLIT && p.lo_r[t] != nullptr, bufA, w, lane,
+      phase_row_product<T, MP, false, NT>(orow, SPLIT && p.lo_r[t] != nullptr, buf, w, lane,
                                           acc);
 #pragma unroll
       for (int u = 0; u < TPW; ++u)
 #pragma unroll
         for (int r = 0; r < 4; ++r) sum[u][r] += acc[u][r];
+    };
+    for (int t = 0; t < nt; t += 2) {
+      const bool two = t + 1 < nt;
+      const T* src0 = ch.in + t * p.tstride;
+      const T* src1 = ch.in + (two ? t + 1 : t) * p.tstride;
+      T x0[KE], x1[KE];
+#pragma unroll
+      for (int k = 0; k < KE; ++k) x0[k] = ok[k] ? src0[g[k]] : T(0);
+      if (two) {
+#pragma unroll
+        for (int k = 0; k < KE; ++k) x1[k] = ok[k] ? src1[g[k]] : T(0);
+      }
+#pragma unroll
+      for (int k = 0; k < KE; ++k) {
+        const int f = tid + NT * k;
+        const int r = f / MP, c = f - r * MP;
+        if (f < MP * MP) {
+          bufA[r * PB + c] = x0[k];
+          if (two) bufB[r * PB + c] = x1[k];
+        }
+      }
+      __syncthreads();
+      add_term(t, bufA);
+      if (two) add_term(t + 1, bufB);
       __syncthreads();
     }
 #pragma unroll
@@ -1224,7 +1248,6 @@ __global__ __launch_bounds__(NT_) void plane_phase_terms_kernel(PhaseTermsParams
     // computes row tile rt and column tile ct of the row factor, then of the
     // z factor ----
     const int rt = (JT / 2) * h + w % (JT / 2), ct = w / (JT / 2);
-    constexpr int KH = (MP / 2 * MP + NT - 1) / NT;
     for (int t = 0; t < nt; ++t) {
       SlabOperands<T, MP, false> orow, ocol;
       slab_load_operands<T, MP, false>(p.mat_r[t], p.lo_r[t], MR, rt, lane, orow);
@@ -1237,20 +1260,16 @@ __global__ __launch_bounds__(NT_) void plane_phase_terms_kernel(PhaseTermsParams
       __syncthreads();
       const V4 z = mfma_tile_a<T, MP, false>(ocol, SPLIT && p.lo_c[t] != nullptr, bufB, PA,
                                              rt * 16 + (lane & 15));
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        lds_put<T>(bufB, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct * 16 + (lane & 15), z[r]);
-      __syncthreads();
+      // the tile straight from registers: 16 contiguous elements per row
       T* out = ch.out + t * p.tstride;
+      const int c = ct * 16 + (lane & 15);
 #pragma unroll
-      for (int k = 0; k < KH; ++k) {
-        const int f = tid + NT * k;
-        const int r = (MP / 2) * h + f / MP, c = f % MP;
-        if (f < MP / 2 * MP && r < MR && c < MC) {
-          const long gi = base + r * RS + c;
-          if constexpr (SPLIT) out[gi] = bufB[r * PB + c];
-          else __builtin_nontemporal_store(bufB[r * PB + c], &out[gi]);
+      for (int r = 0; r < 4; ++r) {
+        const int row = rt * 16 + MfmaOp<T>::row(lane, r);
+        if (row < MR && c < MC) {
+          const long gi = base + row * RS + c;
+          if constexpr (SPLIT) out[gi] = z[r];
+          else __builtin_nontemporal_store(z[r], &out[gi]);
         }
       }
       __syncthreads();  // bufB is rewritten by the next term
