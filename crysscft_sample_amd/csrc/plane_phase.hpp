@@ -119,14 +119,15 @@ __device__ __forceinline__ void phase_row_product(
 #pragma unroll
         for (int u = 0; u < TPW; ++u) {
           if (!on[u]) continue;
-          typename MfmaOp<float>::s4 av, bv;
+          typename MfmaOp<float>::s4 av;
+          float fb[4];
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int n = 4 * c4 + i;
             av[i] = n < KS ? op.alo[n] : (short)0;
-            bv[i] = n < KS ? f32_to_bf16(b[u][n]) : (short)0;
+            fb[i] = n < KS ? b[u][n] : 0.f;
           }
-          acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
+          acc[u] = MfmaOp<float>::mma_bf16(av, bf16x4(fb), acc[u]);
         }
       }
     }
@@ -169,14 +170,15 @@ __device__ __forceinline__ void phase_col_product(
 #pragma unroll
         for (int u = 0; u < TPW; ++u) {
           if (!on[u]) continue;
-          typename MfmaOp<float>::s4 av, bv;
+          typename MfmaOp<float>::s4 bv;
+          float fa[4];
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int n = 4 * c4 + i;
-            av[i] = n < KS ? f32_to_bf16(a[u][n]) : (short)0;
+            fa[i] = n < KS ? a[u][n] : 0.f;
             bv[i] = n < KS ? op.alo[n] : (short)0;
           }
-          acc[u] = MfmaOp<float>::mma_bf16(av, bv, acc[u]);
+          acc[u] = MfmaOp<float>::mma_bf16(bf16x4(fa), bv, acc[u]);
         }
       }
     }
@@ -512,13 +514,14 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_b(const SlabOperands
     if (use_lo) {
 #pragma unroll
       for (int c4 = 0; c4 < KS / 4; ++c4) {
-        typename MfmaOp<float>::s4 av, bv;
+        typename MfmaOp<float>::s4 av;
+        float fb[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           av[i] = op.alo[4 * c4 + i];
-          bv[i] = f32_to_bf16(b[4 * c4 + i]);
+          fb[i] = b[4 * c4 + i];
         }
-        acc = MfmaOp<float>::mma_bf16(av, bv, acc);
+        acc = MfmaOp<float>::mma_bf16(av, bf16x4(fb), acc);
       }
     }
   }
@@ -545,13 +548,14 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a(const SlabOperands
     if (use_lo) {
 #pragma unroll
       for (int c4 = 0; c4 < KS / 4; ++c4) {
-        typename MfmaOp<float>::s4 av, bv;
+        typename MfmaOp<float>::s4 bv;
+        float fa[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          av[i] = f32_to_bf16(a[4 * c4 + i]);
+          fa[i] = a[4 * c4 + i];
           bv[i] = op.alo[4 * c4 + i];
         }
-        acc = MfmaOp<float>::mma_bf16(av, bv, acc);
+        acc = MfmaOp<float>::mma_bf16(bf16x4(fa), bv, acc);
       }
     }
   }
@@ -638,13 +642,14 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_b_khalf(
   // low parts first (see phase_row_product)
   if constexpr (std::is_same<T, float>::value) {
     if (use_lo) {
-      typename MfmaOp<float>::s4 av, bv;
+      typename MfmaOp<float>::s4 av;
+      float fb[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         av[i] = op.alo[i];
-        bv[i] = f32_to_bf16(b[i]);
+        fb[i] = b[i];
       }
-      acc = MfmaOp<float>::mma_bf16(av, bv, acc);
+      acc = MfmaOp<float>::mma_bf16(av, bf16x4(fb), acc);
     }
   }
 #pragma unroll
@@ -664,13 +669,14 @@ __device__ __forceinline__ typename MfmaOp<T>::v4 mfma_tile_a_khalf(
   // low parts first (see phase_row_product)
   if constexpr (std::is_same<T, float>::value) {
     if (use_lo) {
-      typename MfmaOp<float>::s4 av, bv;
+      typename MfmaOp<float>::s4 bv;
+      float fa[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        av[i] = f32_to_bf16(a[i]);
+        fa[i] = a[i];
         bv[i] = op.alo[i];
       }
-      acc = MfmaOp<float>::mma_bf16(av, bv, acc);
+      acc = MfmaOp<float>::mma_bf16(bf16x4(fa), bv, acc);
     }
   }
 #pragma unroll

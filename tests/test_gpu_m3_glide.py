@@ -252,3 +252,30 @@ def test_class64_split_terms_phase_matches_oracle(oracle, g, prec):
         assert dF < 3e-6  # measured 5.9e-7: dense operators, not the half-size products
     run.close()
     s.close()
+
+
+def test_class64_terms_phase_one_workgroup_per_plane(oracle):
+    """CRYS_PHASE_HALF=0 keeps the class-64 multi-term phase on one workgroup
+    per plane; FP64 against the oracle (child process: the switch is read once)."""
+    import json
+    import os
+    import subprocess
+
+    from conftest import ROOT
+
+    N, g = (128, 128, 128), G_CASES[0]
+    red = 0.3 * np.random.default_rng(28).standard_normal(tuple(n // 2 for n in N))
+    tr_ref, *_ = oracle.Solver("3m", N, g=g).scft(0.3, 0.01, 20.0, (4.0, 4.1, 4.3), 0.1, 3,
+                                                  20.0 * red, -20.0 * red)
+    env = dict(os.environ, CRYS_PHASE_HALF="0")
+    r = subprocess.run([sys.executable, "-c", GLIDE_CHILD, ROOT, json.dumps([g, list(N), "fp64"])],
+                       env=env, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    tr = np.array(d["trace"])
+    dF = np.abs(tr[:, 0] / tr_ref[:, 0] - 1).max()
+    dM = np.abs(tr[:, 5] / tr_ref[:, 5] - 1).max()
+    say(f"  3m glide 128^3 CRYS_PHASE_HALF=0: {d['launches']} launch(es) per step, "
+        f"|dF/F| {dF:.2e} |dMaxInc| {dM:.2e}")
+    assert d["launches"] == 1
+    assert dF <= 1e-10 and dM <= 1e-10
