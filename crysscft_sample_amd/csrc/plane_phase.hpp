@@ -908,7 +908,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
           const int col = S8::tile(w, u) * 16 + (lane & 15);
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            lds_put<T>(buf1, (rho * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc8[u][r]);
+            lds_put<T, false>(buf1, (rho * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc8[u][r]);
         }
       }
     } else {
@@ -918,7 +918,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
       const int col = ct * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        lds_put<T>(buf1, (rho * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
+        lds_put<T, false>(buf1, (rho * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
     }
     __syncthreads();
     // ---- E, history rows of this half, E of the next step ----
@@ -965,7 +965,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
         const int col = S8::tile(w, u) * 16 + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc8[u][r]);
+          lds_put<T, false>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc8[u][r]);
       }
     }
   } else if constexpr (REUSE) {
@@ -979,7 +979,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
       const int col = S::tile(w, 0) * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
+        lds_put<T, false>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + col, acc[0][r]);
     }
   } else {
     // k halves over all 16 waves
@@ -988,7 +988,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
     const int rt = kh ? (r2t ^ 1) : r2t;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
+      lds_put<T, false>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
   // ---- rows j, M-1-j of this half (j = 16 h + f / 32) -> z pairs at PA
@@ -1031,7 +1031,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
                         : mfma_tile_a<T, MP, true>(o3f, lo_c, buf0, PA, arow);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        lds_put<T>(buf1, (rt3 * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15),
+        lds_put<T, false>(buf1, (rt3 * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15),
                    acc[r]);
     }
   } else {
@@ -1041,7 +1041,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
     const int rt = kh ? (rt3 ^ 1) : rt3;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
+      lds_put<T, false>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
   // ---- store this half's rows: non-temporal for FP64 as in
