@@ -396,55 +396,14 @@ __global__ __launch_bounds__(kRedThreads) void batch_phi_kernel(const BatchInst<
   __shared__ double sh[kRedThreads];
   const int2 w = work[blockIdx.x];
   const BatchInst<T>& I = insts[w.x];
-  const double ca = scal[4 * (size_t)w.x + 1], cb = scal[4 * (size_t)w.x + 2];
-  const double index = -I.ds / 2, xN = I.xN, acc_mix = I.acc;
-  const long n = I.n;
-  const int NsA = I.NsA, NsB = I.NsB;
+  const PhiFields<T> F{I.qf, I.qb, I.phiA, I.phiB, I.wA, I.wB, I.ksi, I.EA, I.EB,
+                       (size_t)I.n, (size_t)I.NsA, (size_t)I.NsB, scal[4 * (size_t)w.x + 1],
+                       scal[4 * (size_t)w.x + 2], I.xN, I.acc, -I.ds / 2};
   double s_e1 = 0.0, s_enth = 0.0, s_fk = 0.0, m_inc = 0.0;
-  for (long i = (long)w.y * kRedThreads + threadIdx.x; i < n; i += (long)I.nblk * kRedThreads) {
-    // trapezoid sum over slots s0..s1, four slots per round (as
-    // phi_energy_mix_kernel)
-    auto contour_sum = [&](long s0, long s1) -> double {
-      double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
-      long s = s0 + 1;
-      for (; s + 4 <= s1; s += 4) {
-        // each history entry is read once: non-temporal loads
-        const T a0 = __builtin_nontemporal_load(&I.qf[s * n + i]);
-        const T a1 = __builtin_nontemporal_load(&I.qf[(s + 1) * n + i]);
-        const T a2 = __builtin_nontemporal_load(&I.qf[(s + 2) * n + i]);
-        const T a3 = __builtin_nontemporal_load(&I.qf[(s + 3) * n + i]);
-        const T b0 = __builtin_nontemporal_load(&I.qb[s * n + i]);
-        const T b1 = __builtin_nontemporal_load(&I.qb[(s + 1) * n + i]);
-        const T b2 = __builtin_nontemporal_load(&I.qb[(s + 2) * n + i]);
-        const T b3 = __builtin_nontemporal_load(&I.qb[(s + 3) * n + i]);
-        p0 += (double)a0 * (double)b0;
-        p1 += (double)a1 * (double)b1;
-        p2 += (double)a2 * (double)b2;
-        p3 += (double)a3 * (double)b3;
-      }
-      for (; s < s1; ++s) p0 += (double)I.qf[s * n + i] * (double)I.qb[s * n + i];
-      const double ends = 0.5 * ((double)I.qf[s0 * n + i] * (double)I.qb[s0 * n + i] +
-                                 (double)I.qf[s1 * n + i] * (double)I.qb[s1 * n + i]);
-      return (p0 + p1) + (p2 + p3) + ends;
-    };
-    const double pA = ca * contour_sum(0, NsA);
-    const double pB = cb * contour_sum(NsA, NsA + NsB);
-    const double a = I.wA[i], b = I.wB[i], x = I.ksi[i];
-    s_e1 += -a * pA - b * pB;
-    s_enth += xN * pA * pB;
-    const double tt = 1 - pA - pB;
-    s_fk += -x * tt;
-    m_inc = fabs(tt) > m_inc ? fabs(tt) : m_inc;
-    I.phiA[i] = pA;
-    I.phiB[i] = pB;
-    const double na = a + acc_mix * (xN * pB + x - a);
-    const double nb = b + acc_mix * (xN * pA + x - b);
-    I.wA[i] = na;
-    I.wB[i] = nb;
-    I.ksi[i] = 0.5 * (na + nb - xN);
-    I.EA[i] = (T)(1. * exp(index * na));
-    I.EB[i] = (T)(1. * exp(index * nb));
-  }
+  // (the same per-point work as the single-instance kernel, over this
+  // instance's blocks)
+  phi_energy_mix_points<T, false>(F, (size_t)w.y * kRedThreads + threadIdx.x,
+                                  (size_t)I.nblk * kRedThreads, s_e1, s_enth, s_fk, m_inc);
   const double r0 = block_sum(s_e1, sh);
   const double r1 = block_sum(s_enth, sh);
   const double r2 = block_sum(s_fk, sh);

@@ -134,50 +134,61 @@ __global__ __launch_bounds__(kRedThreads) void phi_accumulate_kernel(
   }
 }
 
+// The fields of one SCFT instance that the density/energy/mixing work reads
+// and writes (see phi_energy_mix_points).
+template <typename T>
+struct PhiFields {
+  const T* qf;
+  const T* qb;
+  double *phiA, *phiB, *wA, *wB, *ksi;
+  T *EA, *EB;
+  size_t n, NsA, NsB;
+  double ca, cb, xN, acc_mix, index;
+};
+
 // integrate_phi (:348-360) for both blocks from the joined histories
 // (forward slots 0..NsA are q_A, NsA..NsA+NsB are q_B; backward slots the
 // same for q^dagger), then print_energy's sums (:404-418) with the
 // pre-update fields, calc_new_fields (:362-371), and the next iteration's
-// calc_ksi / array_exp (:478-480).  With FROM_SUMS, phiA / phiB hold the
-// contour sums of phi_accumulate_kernel on entry and qf / qb are not read.
-template <typename T, bool FROM_SUMS = false>
-__global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
-    const T* qf, const T* qb, const double* scal, double* phiA, double* phiB, double* wA,
-    double* wB, double* ksi, T* EA, T* EB, double* part, size_t n, size_t NsA, size_t NsB,
-    double xN, double acc_mix, double ds) {
-  __shared__ double sh[kRedThreads];
-  const double ca = scal[1], cb = scal[2];
-  const double index = -ds / 2;
-  double s_e1 = 0.0, s_enth = 0.0, s_fk = 0.0, m_inc = 0.0;
+// calc_ksi / array_exp (:478-480), for the points of thread t0 of nthr.
+// With FROM_SUMS, phiA / phiB hold the contour sums of phi_accumulate_kernel
+// on entry and qf / qb are not read.
+template <typename T, bool FROM_SUMS>
+__device__ __forceinline__ void phi_energy_mix_points(const PhiFields<T>& F, size_t t0, size_t nthr,
+                                                      double& s_e1, double& s_enth,
+                                                      double& s_fk, double& m_inc) {
+  const size_t n = F.n;
   // the field work of point i from its two contour sums
   auto finish = [&](size_t i, double sA, double sB) {
-    const double pA = ca * (FROM_SUMS ? phiA[i] : sA);
-    const double pB = cb * (FROM_SUMS ? phiB[i] : sB);
-    const double a = wA[i], b = wB[i], x = ksi[i];
+    const double pA = F.ca * (FROM_SUMS ? F.phiA[i] : sA);
+    const double pB = F.cb * (FROM_SUMS ? F.phiB[i] : sB);
+    const double a = F.wA[i], b = F.wB[i], x = F.ksi[i];
     s_e1 += -a * pA - b * pB;
-    s_enth += xN * pA * pB;
+    s_enth += F.xN * pA * pB;
     const double t = 1 - pA - pB;
     s_fk += -x * t;
     m_inc = fabs(t) > m_inc ? fabs(t) : m_inc;
-    phiA[i] = pA;
-    phiB[i] = pB;
-    const double na = a + acc_mix * (xN * pB + x - a);
-    const double nb = b + acc_mix * (xN * pA + x - b);
-    wA[i] = na;
-    wB[i] = nb;
-    ksi[i] = 0.5 * (na + nb - xN);
-    EA[i] = (T)(1. * exp(index * na));
-    EB[i] = (T)(1. * exp(index * nb));
+    F.phiA[i] = pA;
+    F.phiB[i] = pB;
+    const double na = a + F.acc_mix * (F.xN * pB + x - a);
+    const double nb = b + F.acc_mix * (F.xN * pA + x - b);
+    F.wA[i] = na;
+    F.wB[i] = nb;
+    F.ksi[i] = 0.5 * (na + nb - F.xN);
+    F.EA[i] = (T)(1. * exp(F.index * na));
+    F.EB[i] = (T)(1. * exp(F.index * nb));
   };
   // sum_s w_s q[s] q+[s] over slots s0..s1 (trapezoid: the end slots weigh
   // 1/2) for V consecutive points, one 16-byte load per slot and chain; R
   // slots per round with independent partial sums, so that 2 R loads are in
-  // flight per thread (the grid holds 8 waves per CU).  Each history entry is
-  // read once here, non-temporally: the FP64 iteration at 128^3 measured 1.7%
-  // faster than with ordinary loads.  16-byte loads of V points measured
-  // faster than one point per thread with R = 8 (fewer instructions per byte).
+  // flight per thread.  Each history entry is read once here, non-temporally:
+  // the FP64 iteration at 128^3 measured 1.7% faster than with ordinary
+  // loads.  16-byte loads of V points measured faster than one point per
+  // thread (SCFT iteration at 128^3: FP64 0.6%, FP32 1.2%).
   constexpr int V = 16 / (int)sizeof(T);
   typedef T vT __attribute__((ext_vector_type(V)));
+  const T* const qf = F.qf;
+  const T* const qb = F.qb;
   auto contour_sums = [&](size_t i0, size_t s0, size_t s1, double (&out)[V]) {
     constexpr int R = 4;
     double p[R][V];
@@ -210,14 +221,12 @@ __global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
       out[v] = ((p[0][v] + p[1][v]) + (p[2][v] + p[3][v])) +
                0.5 * ((double)a0[v] * (double)b0[v] + (double)a1[v] * (double)b1[v]);
   };
-  const size_t t0 = blockIdx.x * (size_t)kRedThreads + threadIdx.x;
-  const size_t nthr = (size_t)gridDim.x * kRedThreads;
   const size_t nv = n - n % V;  // (the grid rules make n a multiple of 8)
   for (size_t i0 = t0 * V; i0 < nv; i0 += nthr * V) {
     double sA[V], sB[V];
     if constexpr (!FROM_SUMS) {
-      contour_sums(i0, 0, NsA, sA);
-      contour_sums(i0, NsA, NsA + NsB, sB);
+      contour_sums(i0, 0, F.NsA, sA);
+      contour_sums(i0, F.NsA, F.NsA + F.NsB, sB);
     }
 #pragma unroll
     for (int v = 0; v < V; ++v) finish(i0 + v, FROM_SUMS ? 0.0 : sA[v], FROM_SUMS ? 0.0 : sB[v]);
@@ -232,11 +241,26 @@ __global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
         for (size_t s = s0 + 1; s < s1; ++s) t += (double)qf[s * n + i] * (double)qb[s * n + i];
         return t;
       };
-      sA = sum1(0, NsA);
-      sB = sum1(NsA, NsA + NsB);
+      sA = sum1(0, F.NsA);
+      sB = sum1(F.NsA, F.NsA + F.NsB);
     }
     finish(i, sA, sB);
   }
+}
+
+// phi_energy_mix_points over one instance, then the block's partial sums of
+// the energy terms and the incompressibility maximum.
+template <typename T, bool FROM_SUMS = false>
+__global__ __launch_bounds__(kRedThreads) void phi_energy_mix_kernel(
+    const T* qf, const T* qb, const double* scal, double* phiA, double* phiB, double* wA,
+    double* wB, double* ksi, T* EA, T* EB, double* part, size_t n, size_t NsA, size_t NsB,
+    double xN, double acc_mix, double ds) {
+  __shared__ double sh[kRedThreads];
+  const PhiFields<T> F{qf, qb, phiA, phiB, wA, wB, ksi, EA, EB, n, NsA, NsB,
+                       scal[1], scal[2], xN, acc_mix, -ds / 2};
+  double s_e1 = 0.0, s_enth = 0.0, s_fk = 0.0, m_inc = 0.0;
+  phi_energy_mix_points<T, FROM_SUMS>(F, blockIdx.x * (size_t)kRedThreads + threadIdx.x,
+                                      (size_t)gridDim.x * kRedThreads, s_e1, s_enth, s_fk, m_inc);
   const double r0 = block_sum(s_e1, sh);
   const double r1 = block_sum(s_enth, sh);
   const double r2 = block_sum(s_fk, sh);
