@@ -229,10 +229,14 @@ def test_scft_paths_match_oracle(oracle, g, N, phase):
 
 # 128^3 logical (64^3 reduced, class 64): the multi-term phase on two
 # workgroups per plane, each starting the next step for half of the rows.
-@pytest.mark.parametrize("g,prec", [(G_CASES[0], "fp64"), (G_CASES[1], "fp64"),
-                                    (G_CASES[2], "fp64"), (G_CASES[0], "fp32")])
-def test_class64_split_terms_phase_matches_oracle(oracle, g, prec):
-    N = (128, 128, 128)
+@pytest.mark.parametrize("g,prec,N", [(G_CASES[0], "fp64", (128, 128, 128)),
+                                      (G_CASES[1], "fp64", (128, 128, 128)),
+                                      (G_CASES[2], "fp64", (128, 128, 128)),
+                                      (G_CASES[0], "fp32", (128, 128, 128)),
+                                      # 56 x 64 x 64 reduced: the y-slab phases have 56
+                                      # rows in class 64 (padded)
+                                      (G_CASES[0], "fp64", (112, 128, 128))])
+def test_class64_split_terms_phase_matches_oracle(oracle, g, prec, N):
     red = 0.3 * np.random.default_rng(28).standard_normal(tuple(n // 2 for n in N))
     tr_ref, *_ = oracle.Solver("3m", N, g=g).scft(0.3, 0.01, 20.0, (4.0, 4.1, 4.3), 0.1, 3,
                                                   20.0 * red, -20.0 * red)
@@ -243,7 +247,7 @@ def test_class64_split_terms_phase_matches_oracle(oracle, g, prec):
     _, launches = run.time_propagation(reps=1)
     dF = np.abs(tr[:, 0] / tr_ref[:, 0] - 1).max()
     dM = np.abs(tr[:, 5] / tr_ref[:, 5] - 1).max()
-    say(f"  3m glide 128^3 {prec} g={g}: {launches} launch(es) per step, |dF/F| {dF:.2e} "
+    say(f"  3m glide {N} {prec} g={g}: {launches} launch(es) per step, |dF/F| {dF:.2e} "
         f"|dMaxInc| {dM:.2e}")
     assert launches == 1
     if prec == "fp64":
