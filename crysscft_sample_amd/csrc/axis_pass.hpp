@@ -268,9 +268,9 @@ struct MfmaOp<float> {
 // store keeps the stores apart.  (A volatile pointer kept them apart too, but
 // compiled to a flat store of system scope followed by a wait for every
 // outstanding global access.)  KEEP_APART = false lets them merge: the
-// class-64 split phase, whose merged pairs have the right offsets (rows 80
-// dwords apart, 16- and 32-row strides in st64 units), 0.5% faster per FP32
-// step.
+// class-64 split phase's default products (not its k-half variant), whose
+// merged pairs have the right offsets (rows 80 dwords apart, 16- and 32-row
+// strides in st64 units), 0.5% faster per FP32 step.
 template <typename T, bool KEEP_APART = true>
 __device__ __forceinline__ void lds_put(T* p, int idx, T v) {
   p[idx] = v;

@@ -988,8 +988,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
     const int rt = kh ? (r2t ^ 1) : r2t;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      lds_put<T, false>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15),
-                        acc[r]);
+      lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
   // ---- rows j, M-1-j of this half (j = 16 h + f / 32) -> z pairs at PA
@@ -1042,8 +1041,7 @@ __device__ __forceinline__ void plane_phase_half_body(const PhaseParams<T>& p) {
     const int rt = kh ? (rt3 ^ 1) : rt3;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      lds_put<T, false>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15),
-                        acc[r]);
+      lds_put<T>(buf1, (rt * 16 + MfmaOp<T>::row(lane, r)) * PB + ct2 * 16 + (lane & 15), acc[r]);
   }
   __syncthreads();
   // ---- store this half's rows: non-temporal for FP64 as in
