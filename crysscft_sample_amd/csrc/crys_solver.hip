@@ -55,6 +55,10 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 constexpr int kTraceCap = 1024;
+// Iterations per graph launch in crys_scft_run: fewer graph boundaries, 0.2%
+// more SCFT iterations per second at 128^3 (FP64 589.1 -> 590.3, FP32
+// 801.5 -> 803.3 iter/s; 16 measured the same as 4).
+constexpr int kGraphIters = 4;
 
 unsigned grid_for(size_t n) {
   size_t b = (n + 255) / 256;
@@ -201,7 +205,11 @@ struct crys_scft {
   void* ck = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  unsigned long graph_gen = 0;  // sv->ops_gen when the graph was captured
+  // kGraphIters iterations in one graph (run() launches it while that many
+  // remain, then the one-iteration graph)
+  hipGraph_t graph_n = nullptr;
+  hipGraphExec_t exec_n = nullptr;
+  unsigned long graph_gen = 0;  // sv->ops_gen when the graphs were captured
   bool fields_set = false;
 };
 
@@ -1038,11 +1046,19 @@ int crys_propagate_noalloc(crys_solver_t* h, const double* E, const double* init
 // ---------------------------------------------------------------------------
 namespace {
 
-void scft_free(crys_scft* h) {
+void drop_graphs(crys_scft* h) {
   if (h->exec) (void)hipGraphExecDestroy(h->exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
+  if (h->exec_n) (void)hipGraphExecDestroy(h->exec_n);
+  if (h->graph_n) (void)hipGraphDestroy(h->graph_n);
   h->exec = nullptr;
   h->graph = nullptr;
+  h->exec_n = nullptr;
+  h->graph_n = nullptr;
+}
+
+void scft_free(crys_scft* h) {
+  drop_graphs(h);
   void* bufs[] = {h->wA,    h->wB,   h->ksi,   h->phiA,    h->phiB,   h->EA,
                   h->EB,    h->qf,   h->qb,    h->scr[0],  h->scr[1], h->qpart,
                   h->epart, h->scal, h->trace, h->counter, h->gA,     h->gB,
@@ -1299,23 +1315,36 @@ void scft_free_storage(crys_scft* h) {
 
 int scft_build_graph(crys_scft* h) {
   crys_solver* sv = h->sv;
-  if (h->exec) (void)hipGraphExecDestroy(h->exec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  h->exec = nullptr;
-  h->graph = nullptr;
+  drop_graphs(h);
   CHK(scft_storage(h));
-  HIPCHK(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
-  int r = sv->precision == CRYS_FP32 ? scft_record_iteration<float>(h, sv->stream, h->acc)
+  // one iteration, then kGraphIters iterations back to back
+  for (int iters : {1, kGraphIters}) {
+    HIPCHK(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
+    int r = CRYS_OK;
+    for (int i = 0; i < iters && r == CRYS_OK; ++i)
+      r = sv->precision == CRYS_FP32 ? scft_record_iteration<float>(h, sv->stream, h->acc)
                                      : scft_record_iteration<double>(h, sv->stream, h->acc);
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(sv->stream, &g);
-  if (r != CRYS_OK) {
-    if (g) (void)hipGraphDestroy(g);
-    return r;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(sv->stream, &g);
+    if (r != CRYS_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      drop_graphs(h);
+      return r;
+    }
+    if (e != hipSuccess) {
+      drop_graphs(h);
+      return fail(CRYS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    hipGraph_t& slot = iters == 1 ? h->graph : h->graph_n;
+    hipGraphExec_t& ex = iters == 1 ? h->exec : h->exec_n;
+    slot = g;
+    const hipError_t ie = hipGraphInstantiate(&ex, slot, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      ex = nullptr;
+      drop_graphs(h);
+      return fail(CRYS_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+    }
   }
-  if (e != hipSuccess) return fail(CRYS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-  h->graph = g;
-  HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
   h->graph_gen = sv->ops_gen;
   return CRYS_OK;
 }
@@ -1514,10 +1543,7 @@ int crys_scft_set_checkpointing(crys_scft_t* h, size_t every, size_t budget_byte
   if (!h) return fail(CRYS_ERR_INVALID_ARGUMENT, "null handle");
   HIPCHK(hipStreamSynchronize(h->sv->stream));
   // the iteration graph records the storage addresses: rebuild it on next use
-  if (h->exec) (void)hipGraphExecDestroy(h->exec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  h->exec = nullptr;
-  h->graph = nullptr;
+  drop_graphs(h);
   scft_free_storage(h);
   const size_t old_every = h->every_req, old_budget = h->budget;
   h->every_req = every;
@@ -1574,7 +1600,10 @@ int crys_scft_run(crys_scft_t* h, size_t nstep, double* trace_host) {
     const size_t chunk = nstep - done < (size_t)kTraceCap ? nstep - done : (size_t)kTraceCap;
     hipLaunchKernelGGL(crys::set_int_kernel, dim3(1), dim3(1), 0, s, h->counter, 0);
     HIPCHK(hipGetLastError());
-    for (size_t i = 0; i < chunk; ++i) HIPCHK(hipGraphLaunch(h->exec, s));
+    size_t i = 0;
+    if (h->exec_n)
+      for (; i + kGraphIters <= chunk; i += kGraphIters) HIPCHK(hipGraphLaunch(h->exec_n, s));
+    for (; i < chunk; ++i) HIPCHK(hipGraphLaunch(h->exec, s));
     double* dst = trace_host ? trace_host + done * CRYS_TRACE_COLS : rows.data();
     HIPCHK(hipMemcpyAsync(dst, h->trace, chunk * CRYS_TRACE_COLS * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

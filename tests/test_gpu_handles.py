@@ -5,6 +5,8 @@
     cell, also when the solver's operator addresses are reused;
   - a handle keeps its own contour step when the caller gives the solver
     another one (E, NsA and NsB belong to the handle's ds);
+  - a run gives the same bits however its iterations are split over calls
+    (graphs of four iterations and single ones);
   - a non-finite run is reported without a trace buffer too;
   - the one-launch phases on grids whose axes differ inside one length class
     (row axis length != z length, x-slab and y-slab phases with different
@@ -92,6 +94,40 @@ def test_handle_keeps_its_contour_step():
     np.testing.assert_array_equal(t1, r1)
     np.testing.assert_array_equal(t2, r2)
     np.testing.assert_array_equal(t1b, r1[0])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_runs_in_any_split_are_identical(prec):
+    """crys_scft_run launches a graph of four iterations while four remain,
+    then single iterations: 9 iterations at once, 4 + 5, and 9 single calls
+    give bitwise the same trace and fields.  After a cell change both graphs
+    are captured again: 6 iterations then equal a fresh handle's 6."""
+    N, fA, xN, ds, acc = (64, 64, 64), 0.3, 25.0, 0.01, 0.1
+    c1, c2 = (4.2,) * 3, (4.3,) * 3
+    a, b = seeded("PmmmDCT", N, c1, fA, xN, seed=3)
+    runs = []
+    for split in ([9], [4, 5], [1] * 9):
+        s = C.SolverPmmmDCT(N, precision=prec)
+        h = C.SCFT(s, fA, ds, xN, c1, acc)
+        h.set_fields(a, b)
+        tr = np.concatenate([h.run(k) for k in split])
+        runs.append((split, tr, h.fields()))
+        if split == [9]:
+            s.set_cell_para(c2)
+            s.set_contour_step(ds)
+            after = h.run(6)
+            _, _, wA9, wB9 = runs[0][2]
+            fresh = C.SCFT(C.SolverPmmmDCT(N, precision=prec), fA, ds, xN, c2, acc)
+            fresh.set_fields(wA9, wB9)
+            want = fresh.run(6)
+            say(f"  {prec}: 6 iterations after the cell change {rel_err(after, want):.2e}")
+            np.testing.assert_array_equal(after, want)
+    _, tr0, f0 = runs[0]
+    for split, tr, f in runs[1:]:
+        say(f"  {prec}: split {split[:3]}...: trace {rel_err(tr, tr0):.2e}")
+        np.testing.assert_array_equal(tr, tr0)
+        for x, y in zip(f, f0):
+            np.testing.assert_array_equal(x, y)
 
 
 def test_divergence_without_trace():
