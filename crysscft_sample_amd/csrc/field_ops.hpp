@@ -52,30 +52,33 @@ __global__ void prep_fields_kernel(const double* wA, const double* wB, double* k
   }
 }
 
+// Block reductions over kRedThreads threads: a butterfly inside each wave,
+// then the four wave results through LDS, two barriers in all (the LDS tree
+// over 256 values took nine).  Every thread gets the same value, and the
+// order of the additions is fixed, so results repeat bitwise.
+static_assert(kRedThreads == 256, "four waves per reduction block");
+
 __device__ inline double block_sum(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] = sh[t] + sh[t + s];
-    __syncthreads();
-  }
-  double r = sh[0];
+  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
   __syncthreads();
   return r;
 }
 
 __device__ inline double block_max(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] = sh[t] > sh[t + s] ? sh[t] : sh[t + s];
-    __syncthreads();
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double u = __shfl_xor(v, o);
+    v = u > v ? u : v;
   }
-  double r = sh[0];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  return r;
+  const double a = sh[0] > sh[1] ? sh[0] : sh[1], b = sh[2] > sh[3] ? sh[2] : sh[3];
+  __syncthreads();
+  return a > b ? a : b;
 }
 
 // Partial sums of q(s=0) of the backward chain for Q = <q_A^dagger(0)>
